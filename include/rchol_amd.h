@@ -1,0 +1,127 @@
+/*
+ * rchol_amd -- MI355X (gfx950) solve engine for rchol's preconditioned
+ * conjugate gradient.  C ABI: plain pointers and sizes, integer status codes
+ * (0 = success, negative = error; rchol_amd_last_error() has the text), no
+ * exceptions across the boundary.
+ *
+ * Matrices arrive in the reference's SparseCSR arrays (c++/sparse.hpp:26-30):
+ * size_t row pointers, size_t column indices, double values.  The factor is
+ * passed exactly as rchol() stores it in its SparseCSR G: the CSC arrays of
+ * the lower-triangular G (rchol_lap.cpp:358-387), i.e. CSR of G^T.  The
+ * preconditioner is M^-1 = G^-T G^-1 (c++/util/pcg.cpp:141-159).
+ *
+ * Threading: one solve per context at a time; distinct contexts may be used
+ * from different host threads.
+ */
+#ifndef RCHOL_AMD_H
+#define RCHOL_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rchol_amd_ctx rchol_amd_ctx;
+
+/* Status codes. */
+#define RCHOL_AMD_OK 0
+#define RCHOL_AMD_EINVAL (-1)   /* bad argument / inconsistent sizes */
+#define RCHOL_AMD_EHIP (-2)     /* HIP runtime error (no GPU, OOM, launch) */
+#define RCHOL_AMD_ESTATE (-3)   /* matrix or factor not uploaded */
+#define RCHOL_AMD_ERANGE (-4)   /* size beyond the int32 device index range */
+#define RCHOL_AMD_EPIVOT (-5)   /* zero or missing diagonal in G */
+#define RCHOL_AMD_ESTALL (-6)   /* a triangular-solve wait exceeded its bound */
+
+/* Factor storage precision for rchol_amd_upload_G. */
+#define RCHOL_AMD_FP64 0
+#define RCHOL_AMD_FP32 1 /* G values and the two triangular sweeps in FP32 */
+
+/* Option keys for rchol_amd_set_option. */
+#define RCHOL_AMD_OPT_GRAPH_ITERS 1   /* PCG iterations per captured hipGraph (0 = no graph) */
+#define RCHOL_AMD_OPT_RESID_REPLACE 2 /* true-residual replacement period (0 = never) */
+#define RCHOL_AMD_OPT_SPMV_LANES 3    /* lanes per row in SpMV (4, 8, 16 or 32) */
+#define RCHOL_AMD_OPT_TRSV_LANES 4    /* lanes per row in the triangular sweeps */
+
+/* Counters filled by rchol_amd_get_stats. */
+typedef struct rchol_amd_stats {
+  size_t n;
+  size_t nnz_a;
+  size_t nnz_g;            /* stored entries of G including the diagonal */
+  int fwd_levels;          /* dependency levels of G y = r   */
+  int bwd_levels;          /* dependency levels of G^T z = y */
+  int fwd_tasks;
+  int bwd_tasks;
+  int trsv_waves;          /* resident waves of the persistent sweeps */
+  int g_precision;         /* RCHOL_AMD_FP64 / RCHOL_AMD_FP32 */
+  int last_iters;
+  double last_solve_ms;    /* device time of the last PCG loop (events) */
+  double last_relres;
+} rchol_amd_stats;
+
+/* Thread-local text of the last error on the calling thread. */
+const char *rchol_amd_last_error(void);
+
+/* Context: one HIP device and one stream.  No reference counterpart (the
+ * reference builds its MKL handles inside every pcg() call, pcg.cpp:22-27). */
+int rchol_amd_create(int device, rchol_amd_ctx **ctx);
+int rchol_amd_destroy(rchol_amd_ctx *ctx);
+
+/* Device residency of A.  Replaces pcg::create_sparse for A
+ * (c++/util/pcg.cpp:22, :31-54). */
+int rchol_amd_upload_A(rchol_amd_ctx *ctx, size_t n, const size_t *rowPtr,
+                       const size_t *colIdx, const double *val);
+
+/* Device residency of the factor, from the arrays rchol() stores
+ * (rchol_lap.cpp:358-387): colPtr[n+1], rowIdx[nnz], val[nnz].  Builds the
+ * forward (G) and backward (G^T) sweeps with their dependency-level
+ * schedules.  Replaces pcg::create_sparse for G (c++/util/pcg.cpp:23,
+ * :31-54) and the TRIANGULAR/UPPER descriptor of pcg.cpp:146-149. */
+int rchol_amd_upload_G(rchol_amd_ctx *ctx, size_t n, const size_t *colPtr,
+                       const size_t *rowIdx, const double *val, int precision);
+
+/* z = G^-T G^-1 r on host buffers.  Replaces pcg::precond_solve
+ * (c++/util/pcg.cpp:141-159). */
+int rchol_amd_precond(rchol_amd_ctx *ctx, const double *r, double *z);
+
+/* y = A x on host buffers.  Replaces pcg::matrix_vector_product
+ * (c++/util/pcg.cpp:130-138). */
+int rchol_amd_spmv(rchol_amd_ctx *ctx, const double *x, double *y);
+
+/* PCG on host buffers.  Replaces pcg::iteration (c++/util/pcg.cpp:57-127):
+ * r0 = b, stop when ||r||_2 <= tol*||b||_2 or after maxit iterations,
+ * alpha = (p.r)/(p.q), beta = (r.z)/(r_prev.z_prev), relres = ||Ax-b||/||b||
+ * on the true residual.  x (length n) is read as the starting accumulator
+ * when x_in_is_initial != 0 (the reference keeps a pre-sized caller x,
+ * pcg.cpp:67, while r still starts at b), else treated as zeros. */
+int rchol_amd_pcg(rchol_amd_ctx *ctx, const double *b, double tol, int maxit, double *x,
+                  int x_in_is_initial, double *relres, int *itr);
+
+/* Same on device pointers (d_b, d_x on the context's device; d_x is
+ * overwritten).  Used by the Python binding with torch tensors. */
+int rchol_amd_pcg_device(rchol_amd_ctx *ctx, const double *d_b, double tol, int maxit,
+                         double *d_x, int x_in_is_initial, double *relres, int *itr);
+
+/* Preconditioner apply on device pointers. */
+int rchol_amd_precond_device(rchol_amd_ctx *ctx, const double *d_r, double *d_z);
+
+/* One-shot solve with the exact argument list of the reference constructor
+ * pcg(A, b, tol, maxit, G, x, relres, itr) (c++/util/pcg.hpp:15-16,
+ * pcg.cpp:14-28): A as SparseCSR arrays, G as the stored factor arrays.
+ * Uses device 0; creates and destroys a context per call. */
+int rchol_amd_pcg_csr(size_t n, const size_t *aRowPtr, const size_t *aColIdx,
+                      const double *aVal, const double *b, double tol, int maxit,
+                      const size_t *gColPtr, const size_t *gRowIdx, const double *gVal,
+                      double *x, int x_in_is_initial, double *relres, int *itr);
+
+int rchol_amd_set_option(rchol_amd_ctx *ctx, int key, double value);
+int rchol_amd_get_stats(rchol_amd_ctx *ctx, rchol_amd_stats *out);
+
+/* Number of visible HIP devices (0 when none); never fails. */
+int rchol_amd_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RCHOL_AMD_H */

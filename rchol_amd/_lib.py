@@ -1,0 +1,88 @@
+"""ctypes loader for the in-tree shared libraries.
+
+librchol_amd.so    -- the HIP solve engine (include/rchol_amd.h)
+librchol_factor.so -- host factorization and ordering (include/rchol_factor.h)
+
+Both are built in-tree by ``make -C rchol_amd/csrc`` (``__graft_entry__.build``).
+A missing library raises immediately: there is no fallback path.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ENGINE_PATH = os.path.join(_HERE, "librchol_amd.so")
+FACTOR_PATH = os.path.join(_HERE, "librchol_factor.so")
+
+u64p = ctypes.POINTER(ctypes.c_uint64)
+f64p = ctypes.POINTER(ctypes.c_double)
+i32p = ctypes.POINTER(ctypes.c_int)
+
+# Every C-ABI symbol of include/rchol_amd.h with its signature.
+ENGINE_SYMBOLS = {
+    "rchol_amd_last_error": (ctypes.c_char_p, []),
+    "rchol_amd_device_count": (ctypes.c_int, []),
+    "rchol_amd_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "rchol_amd_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "rchol_amd_upload_A": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, u64p, u64p, f64p]),
+    "rchol_amd_upload_G": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, u64p, u64p, f64p,
+                                          ctypes.c_int]),
+    "rchol_amd_precond": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p]),
+    "rchol_amd_precond_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_void_p]),
+    "rchol_amd_spmv": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p]),
+    "rchol_amd_pcg": (ctypes.c_int, [ctypes.c_void_p, f64p, ctypes.c_double, ctypes.c_int, f64p,
+                                     ctypes.c_int, f64p, i32p]),
+    "rchol_amd_pcg_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                            ctypes.c_int, ctypes.c_void_p, ctypes.c_int, f64p,
+                                            i32p]),
+    "rchol_amd_pcg_csr": (ctypes.c_int, [ctypes.c_size_t, u64p, u64p, f64p, f64p,
+                                         ctypes.c_double, ctypes.c_int, u64p, u64p, f64p, f64p,
+                                         ctypes.c_int, f64p, i32p]),
+    "rchol_amd_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]),
+    "rchol_amd_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+}
+
+FACTOR_SYMBOLS = {}
+
+_engine = None
+_factor = None
+
+
+def _bind(lib, table):
+    for name, (res, args) in table.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def engine():
+    """The HIP engine library (loaded once)."""
+    global _engine
+    if _engine is None:
+        if not os.path.exists(ENGINE_PATH):
+            raise RuntimeError(
+                f"{ENGINE_PATH} is not built; run `make -C rchol_amd/csrc` "
+                "(or __graft_entry__.build()) -- there is no CPU fallback")
+        lib = ctypes.CDLL(ENGINE_PATH)
+        _bind(lib, ENGINE_SYMBOLS)
+        _engine = lib
+    return _engine
+
+
+def factor():
+    """The host factorization library (loaded once)."""
+    global _factor
+    if _factor is None:
+        if not os.path.exists(FACTOR_PATH):
+            raise RuntimeError(f"{FACTOR_PATH} is not built; run `make -C rchol_amd/csrc`")
+        lib = ctypes.CDLL(FACTOR_PATH)
+        _bind(lib, FACTOR_SYMBOLS)
+        _factor = lib
+    return _factor
+
+
+def check(rc, lib=None):
+    if rc != 0:
+        lib = lib or engine()
+        msg = lib.rchol_amd_last_error()
+        raise RuntimeError(f"rchol_amd error {rc}: {msg.decode() if msg else ''}")

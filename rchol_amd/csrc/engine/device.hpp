@@ -1,0 +1,174 @@
+// Device-side building blocks shared by the solve kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rchol_amd {
+
+// Control block in device memory.  The whole PCG state that the host would
+// otherwise hold (alpha, beta, norms, counters) lives here so that an
+// iteration never needs the host.
+struct DevScalars {
+  double bb;        // ||b||^2
+  double rr;        // ||r||^2 of the recursively updated residual
+  double rz;        // r.z of the current iteration
+  double rz_old;    // r.z of the previous iteration (pcg.cpp:94 d2)
+  double pr;        // p.r  (pcg.cpp:101)
+  double pq;        // p.q  (pcg.cpp:102)
+  double alpha;
+  double beta;
+  double tol;
+  double relres;
+  double qq;        // ||A x - b||^2 (final true residual)
+  int k;            // completed iterations
+  int maxit;
+  int done;         // 1 once the loop test of pcg.cpp:82 fails
+  int err;          // bit 0: a triangular-solve wait hit its bound
+  int zero;         // constant 0: the stop flag of a standalone sweep
+  int resid_period; // true-residual replacement period (0 = off)
+  unsigned int ctr[8];  // last-block counters of the reductions (self-resetting)
+};
+
+enum { CTR_INIT = 0, CTR_BWD = 1, CTR_SPMVP = 2, CTR_UPD = 3, CTR_RES = 4, CTR_GEN = 5 };
+
+// Sentinel bit patterns: signalling NaNs that arithmetic never produces.
+constexpr unsigned long long kSent64 = 0x7FF4DEADBEEFCAFEull;
+constexpr unsigned int kSent32 = 0x7FA5DEADu;
+
+template <typename T> struct Sent;
+template <> struct Sent<double> {
+  static __device__ __forceinline__ double value() { return __longlong_as_double((long long)kSent64); }
+  static __device__ __forceinline__ bool is(double v) {
+    return (unsigned long long)__double_as_longlong(v) == kSent64;
+  }
+  static __device__ __forceinline__ double load_agent(const double *p) {
+    unsigned long long b = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)b);
+  }
+  static __device__ __forceinline__ void store_agent(double *p, double v) {
+    __hip_atomic_store((unsigned long long *)p, (unsigned long long)__double_as_longlong(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+};
+template <> struct Sent<float> {
+  static __device__ __forceinline__ float value() { return __uint_as_float(kSent32); }
+  static __device__ __forceinline__ bool is(float v) { return __float_as_uint(v) == kSent32; }
+  static __device__ __forceinline__ float load_agent(const float *p) {
+    unsigned int b = __hip_atomic_load((const unsigned int *)p, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+    return __uint_as_float(b);
+  }
+  static __device__ __forceinline__ void store_agent(float *p, float v) {
+    __hip_atomic_store((unsigned int *)p, __float_as_uint(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+};
+
+// Bound on one dependency wait, in wall-clock ticks (the constant-rate
+// counter runs at ~100 MHz: ~0.67 s), far beyond any legitimate wait.  A
+// broken schedule therefore ends as an error flag, never as a hang; once the
+// flag is up every other wait returns at its next check.
+constexpr long long kWaitTicks = 1ll << 26;
+
+// Value of x[j] once its producer has published it.
+template <typename T>
+__device__ __forceinline__ T wait_ready(const T *p, int *err) {
+  T v = *p;
+  if (!Sent<T>::is(v)) return v;
+  const long long t0 = wall_clock64();
+  unsigned int polls = 0;
+  do {
+    __builtin_amdgcn_s_sleep(2);
+    v = Sent<T>::load_agent(p);
+    if ((++polls & 127u) == 0) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      if (wall_clock64() - t0 > kWaitTicks) {
+        atomicOr(err, 1);
+        break;
+      }
+    }
+  } while (Sent<T>::is(v));
+  return v;
+}
+
+// Sum over the L consecutive lanes of a row group (L a power of two <= 64).
+template <int L>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, L);
+  return v;
+}
+template <int L>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, L);
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block-wide sum of NV doubles (blockDim.x a multiple of 64, <= 1024);
+// the result is valid in thread 0.  Deterministic for a fixed launch shape.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+  __shared__ double red[NV][16];
+#pragma unroll
+  for (int i = 0; i < NV; i++) v[i] = wave_sum(v[i]);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) red[i][w] = v[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+      double s = 0.0;
+      for (int k = 0; k < nw; k++) s += red[i][k];
+      v[i] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Last-block reduction: every block stores its NV partial sums; the block
+// that arrives last sums all partials in block order (deterministic) and
+// returns true with the totals in tot[] (thread 0 only).  The counter resets
+// itself, so the same kernel can be replayed from a captured graph.
+template <int NV>
+__device__ __forceinline__ bool grid_reduce(double (&v)[NV], double *partials, unsigned int *ctr,
+                                            double (&tot)[NV]) {
+  block_sum<NV>(v);
+  __shared__ int is_last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) partials[(size_t)blockIdx.x * NV + i] = v[i];
+    unsigned int prev =
+        __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    is_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __atomic_thread_fence(__ATOMIC_ACQUIRE);
+  double s[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) s[i] = 0.0;
+  for (unsigned int b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) s[i] += partials[(size_t)b * NV + i];
+  }
+  block_sum<NV>(s);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) tot[i] = s[i];
+    __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return threadIdx.x == 0;
+}
+
+}  // namespace rchol_amd

@@ -1,0 +1,762 @@
+// rchol_amd engine: device residency, schedule analysis, PCG driver and the
+// C ABI declared in include/rchol_amd.h.
+//
+// Reference path replaced (c++/util/pcg.cpp):
+//   pcg::pcg / create_sparse      -> rchol_amd_create / upload_A / upload_G
+//   pcg::iteration                -> Engine::pcg (device-resident loop)
+//   pcg::matrix_vector_product    -> k_spmv_p / k_spmv_res
+//   pcg::precond_solve            -> two k_sptrsv sweeps
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../../include/rchol_amd.h"
+#include "kernels.hpp"
+
+using namespace rchol_amd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(RCHOL_AMD_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_));        \
+  } while (0)
+
+constexpr int kTaskRows = 64;     // slots per triangular-sweep task
+constexpr int kBlock = 256;       // threads per block for every kernel
+constexpr size_t kIdxMax = 0x7FFFFFFFu;
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    reset();
+    n = count;
+    return hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
+  }
+  hipError_t upload(const T *h, size_t count) {
+    hipError_t e = alloc(count);
+    if (e != hipSuccess) return e;
+    return count ? hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice) : hipSuccess;
+  }
+};
+
+// One triangular sweep on the device: rows in level ("slot") order.
+struct Sweep {
+  int nslots = 0, ntasks = 0, nlevels = 0;
+  size_t nnz_off = 0;
+  DevBuf<int> task_ptr, slot_row, rp, ci;
+  DevBuf<double> cv64, diag64;
+  DevBuf<float> cv32, diag32;
+};
+
+// Host-side schedule of one sweep before upload.
+struct HostSweep {
+  std::vector<int> task_ptr, slot_row, rp, ci;
+  std::vector<double> cv, diag;
+  int nlevels = 0;
+};
+
+// Counting sort of rows by level, then level-chunked tasks.
+void build_slots(const std::vector<int> &lev, int nlev, HostSweep &hs, std::vector<int> &slot_of) {
+  const int n = (int)lev.size();
+  std::vector<int> cnt(nlev + 1, 0);
+  for (int i = 0; i < n; i++) cnt[lev[i] + 1]++;
+  for (int l = 0; l < nlev; l++) cnt[l + 1] += cnt[l];
+  hs.slot_row.assign(n, 0);
+  slot_of.assign(n, 0);
+  std::vector<int> cur(cnt.begin(), cnt.end() - 1);
+  for (int i = 0; i < n; i++) {
+    int s = cur[lev[i]]++;
+    hs.slot_row[s] = i;
+    slot_of[i] = s;
+  }
+  hs.task_ptr.clear();
+  for (int l = 0; l < nlev; l++) {
+    for (int s = cnt[l]; s < cnt[l + 1]; s += kTaskRows) hs.task_ptr.push_back(s);
+  }
+  hs.task_ptr.push_back(n);
+  hs.nlevels = nlev;
+}
+
+}  // namespace
+
+struct rchol_amd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int ncu = 0;
+  size_t n = 0;
+  // A in CSR, int32 indices
+  bool hasA = false;
+  size_t nnzA = 0;
+  DevBuf<int> arp, aci;
+  DevBuf<double> av;
+  // G
+  bool hasG = false;
+  size_t nG = 0, nnzG = 0;
+  int gprec = RCHOL_AMD_FP64;
+  Sweep fwd, bwd;
+  int trsv_grid = 0;
+  // work vectors
+  size_t nwork = 0;
+  DevBuf<double> b, x, r, q, p0, p1, partials, tmp64a;
+  DevBuf<double> y64, z64;
+  DevBuf<float> y32, z32;
+  DevScalars *sc = nullptr;
+  DevScalars *hsc = nullptr;  // pinned mirror
+  // options
+  int graph_iters = 16;
+  int resid_period = 0;
+  int spmv_lanes = 8;
+  int trsv_lanes = 8;
+  // captured iteration graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  rchol_amd_stats stats{};
+
+  ~rchol_amd_ctx() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (sc) (void)hipFree(sc);
+    if (hsc) (void)hipHostFree(hsc);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
+
+  int vec_grid() const {
+    size_t g = (n + kBlock - 1) / kBlock;
+    return (int)std::max<size_t>(1, std::min<size_t>(g, (size_t)ncu * 8));
+  }
+  int spmv_grid() const {
+    size_t rows_per_block = (kBlock / 64) * (64 / spmv_lanes);
+    size_t g = (n + rows_per_block - 1) / rows_per_block;
+    return (int)std::max<size_t>(1, std::min<size_t>(g, (size_t)ncu * 8));
+  }
+  int ensure_work();
+  int size_trsv_grid();
+  int launch_iteration(int parity);
+  int pcg(const double *d_b, double tol, int maxit, double *d_x_out, int keep_x, double *relres,
+          int *itr);
+  int precond(const double *d_r, double *d_z);
+};
+
+namespace {
+
+// Launch the forward + backward sweeps.  pcg_mode: fused sentinel/dot
+// epilogues for the PCG loop; otherwise a standalone apply z = M^-1 r.
+template <typename VT, typename XT, int L>
+int launch_sweeps_t(rchol_amd_ctx *c, const double *rhs, XT *y, XT *z, const int *stop,
+                    bool with_dot, const double *dotv) {
+  const VT *fcv, *fdg, *bcv, *bdg;
+  if constexpr (sizeof(VT) == 8) {
+    fcv = c->fwd.cv64.p; fdg = c->fwd.diag64.p; bcv = c->bwd.cv64.p; bdg = c->bwd.diag64.p;
+  } else {
+    fcv = c->fwd.cv32.p; fdg = c->fwd.diag32.p; bcv = c->bwd.cv32.p; bdg = c->bwd.diag32.p;
+  }
+  hipLaunchKernelGGL((k_sptrsv<VT, XT, double, L, 1>), dim3(c->trsv_grid), dim3(kBlock), 0,
+                     c->stream, c->fwd.ntasks, c->fwd.task_ptr.p, c->fwd.slot_row.p, c->fwd.rp.p,
+                     c->fwd.ci.p, fcv, fdg, rhs, y, z, nullptr, nullptr, c->sc, stop);
+  if (with_dot) {
+    hipLaunchKernelGGL((k_sptrsv<VT, XT, XT, L, 2>), dim3(c->trsv_grid), dim3(kBlock), 0,
+                       c->stream, c->bwd.ntasks, c->bwd.task_ptr.p, c->bwd.slot_row.p,
+                       c->bwd.rp.p, c->bwd.ci.p, bcv, bdg, (const XT *)y, z, (XT *)nullptr, dotv,
+                       c->partials.p, c->sc, stop);
+  } else {
+    hipLaunchKernelGGL((k_sptrsv<VT, XT, XT, L, 0>), dim3(c->trsv_grid), dim3(kBlock), 0,
+                       c->stream, c->bwd.ntasks, c->bwd.task_ptr.p, c->bwd.slot_row.p,
+                       c->bwd.rp.p, c->bwd.ci.p, bcv, bdg, (const XT *)y, z, (XT *)nullptr,
+                       nullptr, nullptr, c->sc, stop);
+  }
+  return RCHOL_AMD_OK;
+}
+
+template <typename VT, typename XT>
+int dispatch_sweeps(rchol_amd_ctx *c, const double *rhs, XT *y, XT *z, const int *stop,
+                    bool with_dot, const double *dotv) {
+  switch (c->trsv_lanes) {
+    case 4: return launch_sweeps_t<VT, XT, 4>(c, rhs, y, z, stop, with_dot, dotv);
+    case 16: return launch_sweeps_t<VT, XT, 16>(c, rhs, y, z, stop, with_dot, dotv);
+    case 32: return launch_sweeps_t<VT, XT, 32>(c, rhs, y, z, stop, with_dot, dotv);
+    default: return launch_sweeps_t<VT, XT, 8>(c, rhs, y, z, stop, with_dot, dotv);
+  }
+}
+
+template <typename ZT, int L>
+void launch_spmv_p_t(rchol_amd_ctx *c, const ZT *z, const double *p_old, double *p_new) {
+  hipLaunchKernelGGL((k_spmv_p<ZT, L>), dim3(c->spmv_grid()), dim3(kBlock), 0, c->stream,
+                     (int)c->n, c->arp.p, c->aci.p, c->av.p, z, p_old, p_new, c->q.p, c->r.p,
+                     c->partials.p, c->sc);
+}
+
+template <typename ZT>
+void launch_spmv_p(rchol_amd_ctx *c, const ZT *z, const double *p_old, double *p_new) {
+  switch (c->spmv_lanes) {
+    case 4: launch_spmv_p_t<ZT, 4>(c, z, p_old, p_new); break;
+    case 16: launch_spmv_p_t<ZT, 16>(c, z, p_old, p_new); break;
+    case 32: launch_spmv_p_t<ZT, 32>(c, z, p_old, p_new); break;
+    default: launch_spmv_p_t<ZT, 8>(c, z, p_old, p_new); break;
+  }
+}
+
+template <int L>
+void launch_spmv_res_t(rchol_amd_ctx *c, const double *x, const double *b, double *y) {
+  hipLaunchKernelGGL((k_spmv_res<L>), dim3(c->spmv_grid()), dim3(kBlock), 0, c->stream,
+                     (int)c->n, c->arp.p, c->aci.p, c->av.p, x, b, y, c->partials.p, c->sc);
+}
+
+void launch_spmv_res(rchol_amd_ctx *c, const double *x, const double *b, double *y) {
+  switch (c->spmv_lanes) {
+    case 4: launch_spmv_res_t<4>(c, x, b, y); break;
+    case 16: launch_spmv_res_t<16>(c, x, b, y); break;
+    case 32: launch_spmv_res_t<32>(c, x, b, y); break;
+    default: launch_spmv_res_t<8>(c, x, b, y); break;
+  }
+}
+
+}  // namespace
+
+// Persistent sweep grid: every block must be resident at once (the
+// deadlock-freedom argument of k_sptrsv).  The occupancy is the minimum
+// over the three epilogue variants at the configured lane count.
+template <typename VT, typename XT, int L>
+static int trsv_occupancy() {
+  int a = 0, b = 0, c = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, (const void *)k_sptrsv<VT, XT, double, L, 1>,
+                                                   kBlock, 0) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void *)k_sptrsv<VT, XT, XT, L, 2>,
+                                                   kBlock, 0) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, (const void *)k_sptrsv<VT, XT, XT, L, 0>,
+                                                   kBlock, 0) != hipSuccess)
+    return 0;
+  return std::min(a, std::min(b, c));
+}
+
+template <typename VT, typename XT>
+static int trsv_occupancy_l(int lanes) {
+  switch (lanes) {
+    case 4: return trsv_occupancy<VT, XT, 4>();
+    case 16: return trsv_occupancy<VT, XT, 16>();
+    case 32: return trsv_occupancy<VT, XT, 32>();
+    default: return trsv_occupancy<VT, XT, 8>();
+  }
+}
+
+int rchol_amd_ctx::size_trsv_grid() {
+  int per_cu = gprec == RCHOL_AMD_FP32 ? trsv_occupancy_l<float, float>(trsv_lanes)
+                                       : trsv_occupancy_l<double, double>(trsv_lanes);
+  if (per_cu <= 0) return fail(RCHOL_AMD_EHIP, "occupancy query for the sweep kernels failed");
+  per_cu = std::min(per_cu, 8);
+  const int max_tasks = std::max(fwd.ntasks, bwd.ntasks);
+  trsv_grid = std::max(1, std::min(per_cu * ncu, (max_tasks + 3) / 4));
+  stats.trsv_waves = trsv_grid * (kBlock / 64);
+  return RCHOL_AMD_OK;
+}
+
+int rchol_amd_ctx::ensure_work() {
+  if (nwork == n && b.p) return RCHOL_AMD_OK;
+  drop_graph();
+  HIP_TRY(b.alloc(n));
+  HIP_TRY(x.alloc(n));
+  HIP_TRY(r.alloc(n));
+  HIP_TRY(q.alloc(n));
+  HIP_TRY(p0.alloc(n));
+  HIP_TRY(p1.alloc(n));
+  HIP_TRY(tmp64a.alloc(n));
+  HIP_TRY(y64.alloc(n));
+  HIP_TRY(z64.alloc(n));
+  HIP_TRY(y32.alloc(n));
+  HIP_TRY(z32.alloc(n));
+  HIP_TRY(partials.alloc((size_t)ncu * 8 * 4 + 64));
+  nwork = n;
+  return RCHOL_AMD_OK;
+}
+
+// One PCG iteration (pcg.cpp:82-110 body): M^-1 r, p update + A p + alpha,
+// x/r update + loop test.  parity selects the p ping-pong buffers.
+int rchol_amd_ctx::launch_iteration(int parity) {
+  double *p_old = parity ? p1.p : p0.p;
+  double *p_new = parity ? p0.p : p1.p;
+  const int *stop = &sc->done;
+  if (gprec == RCHOL_AMD_FP32) {
+    dispatch_sweeps<float, float>(this, r.p, y32.p, z32.p, stop, true, r.p);
+    launch_spmv_p<float>(this, z32.p, p_old, p_new);
+    hipLaunchKernelGGL((k_update<float>), dim3(vec_grid()), dim3(kBlock), 0, stream, (int)n,
+                       (const double *)p_new, (const double *)q.p, x.p, r.p, y32.p, partials.p, sc);
+  } else {
+    dispatch_sweeps<double, double>(this, r.p, y64.p, z64.p, stop, true, r.p);
+    launch_spmv_p<double>(this, z64.p, p_old, p_new);
+    hipLaunchKernelGGL((k_update<double>), dim3(vec_grid()), dim3(kBlock), 0, stream, (int)n,
+                       (const double *)p_new, (const double *)q.p, x.p, r.p, y64.p, partials.p,
+                       sc);
+  }
+  return RCHOL_AMD_OK;
+}
+
+int rchol_amd_ctx::pcg(const double *d_b, double tol, int maxit, double *d_x_out, int keep_x,
+                       double *relres, int *itr) {
+  if (!hasA || !hasG) return fail(RCHOL_AMD_ESTATE, "A and G must be uploaded before pcg");
+  if (nG != n) return fail(RCHOL_AMD_EINVAL, "G is %zu x %zu but A is %zu x %zu", nG, nG, n, n);
+  int rc = ensure_work();
+  if (rc) return rc;
+  if (d_x_out != x.p && keep_x)
+    HIP_TRY(hipMemcpyAsync(x.p, d_x_out, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  // scalar set-up
+  hsc->tol = tol;
+  hsc->maxit = maxit;
+  hsc->zero = 0;
+  hsc->resid_period = resid_period;
+  HIP_TRY(hipMemcpyAsync(&sc->tol, &hsc->tol, sizeof(double), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(&sc->maxit, &hsc->maxit, sizeof(int), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(&sc->zero, &hsc->zero, sizeof(int), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipEventRecord(ev0, stream));
+  if (gprec == RCHOL_AMD_FP32)
+    hipLaunchKernelGGL((k_init<float>), dim3(vec_grid()), dim3(kBlock), 0, stream, (int)n, d_b,
+                       r.p, x.p, keep_x, y32.p, partials.p, sc);
+  else
+    hipLaunchKernelGGL((k_init<double>), dim3(vec_grid()), dim3(kBlock), 0, stream, (int)n, d_b,
+                       r.p, x.p, keep_x, y64.p, partials.p, sc);
+  HIP_TRY(hipGetLastError());
+
+  int K = graph_iters > 0 ? graph_iters : 8;
+  if (K & 1) K++;
+  if (graph_iters > 0 && !gexec) {
+    HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < K; i++) launch_iteration(i & 1);
+    hipError_t e = hipStreamEndCapture(stream, &graph);
+    if (e != hipSuccess) return fail(RCHOL_AMD_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+    HIP_TRY(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+  }
+  for (;;) {
+    if (gexec) {
+      HIP_TRY(hipGraphLaunch(gexec, stream));
+    } else {
+      for (int i = 0; i < K; i++) launch_iteration(i & 1);
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemcpyAsync(hsc, sc, sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (hsc->err) break;
+    if (hsc->done) break;
+  }
+  if (hsc->err)
+    return fail(RCHOL_AMD_ESTALL, "triangular sweep wait exceeded its bound (err=%d)", hsc->err);
+  // the last completed iteration wrote p_new into p1 when k is odd
+  launch_spmv_res(this, x.p, d_b, tmp64a.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ev1, stream));
+  HIP_TRY(hipMemcpyAsync(hsc, sc, sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
+  if (d_x_out != x.p)
+    HIP_TRY(hipMemcpyAsync(d_x_out, x.p, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+  *relres = hsc->relres;
+  *itr = hsc->k;
+  stats.last_iters = hsc->k;
+  stats.last_relres = hsc->relres;
+  stats.last_solve_ms = ms;
+  return RCHOL_AMD_OK;
+}
+
+int rchol_amd_ctx::precond(const double *d_r, double *d_z) {
+  if (!hasG) return fail(RCHOL_AMD_ESTATE, "G must be uploaded before precond");
+  if (!hasA) n = nG;
+  if (n != nG) return fail(RCHOL_AMD_EINVAL, "G is %zu x %zu but A is %zu x %zu", nG, nG, n, n);
+  int rc = ensure_work();
+  if (rc) return rc;
+  hsc->zero = 0;
+  hsc->err = 0;
+  HIP_TRY(hipMemcpyAsync(&sc->zero, &hsc->zero, sizeof(int), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(&sc->err, &hsc->err, sizeof(int), hipMemcpyHostToDevice, stream));
+  const int *stop = &sc->zero;
+  const int g = vec_grid();
+  if (gprec == RCHOL_AMD_FP32) {
+    hipLaunchKernelGGL((k_fill_sentinel<float>), dim3(g), dim3(kBlock), 0, stream, (int)n, y32.p);
+    dispatch_sweeps<float, float>(this, d_r, y32.p, z32.p, stop, false, nullptr);
+    hipLaunchKernelGGL(k_widen, dim3(g), dim3(kBlock), 0, stream, (int)n, (const float *)z32.p, d_z);
+  } else {
+    hipLaunchKernelGGL((k_fill_sentinel<double>), dim3(g), dim3(kBlock), 0, stream, (int)n, y64.p);
+    dispatch_sweeps<double, double>(this, d_r, y64.p, z64.p, stop, false, nullptr);
+    HIP_TRY(hipMemcpyAsync(d_z, z64.p, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(&hsc->err, &sc->err, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  if (hsc->err)
+    return fail(RCHOL_AMD_ESTALL, "triangular sweep wait exceeded its bound (err=%d)", hsc->err);
+  return RCHOL_AMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *rchol_amd_last_error(void) { return g_err.c_str(); }
+
+int rchol_amd_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int rchol_amd_create(int device, rchol_amd_ctx **out) {
+  if (!out) return fail(RCHOL_AMD_EINVAL, "null output pointer");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(RCHOL_AMD_EHIP, "no HIP device is visible");
+  if (device < 0 || device >= ndev)
+    return fail(RCHOL_AMD_EINVAL, "device %d out of range (%d visible)", device, ndev);
+  std::unique_ptr<rchol_amd_ctx> c(new rchol_amd_ctx());
+  c->device = device;
+  HIP_TRY(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  c->ncu = prop.multiProcessorCount;
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIP_TRY(hipMalloc((void **)&c->sc, sizeof(DevScalars)));
+  HIP_TRY(hipMemset(c->sc, 0, sizeof(DevScalars)));
+  HIP_TRY(hipHostMalloc((void **)&c->hsc, sizeof(DevScalars), hipHostMallocDefault));
+  memset(c->hsc, 0, sizeof(DevScalars));
+  HIP_TRY(hipEventCreate(&c->ev0));
+  HIP_TRY(hipEventCreate(&c->ev1));
+  *out = c.release();
+  return RCHOL_AMD_OK;
+}
+
+int rchol_amd_destroy(rchol_amd_ctx *ctx) {
+  if (!ctx) return RCHOL_AMD_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+  return RCHOL_AMD_OK;
+}
+
+int rchol_amd_upload_A(rchol_amd_ctx *c, size_t n, const size_t *rowPtr, const size_t *colIdx,
+                       const double *val) {
+  if (!c || !rowPtr || (!colIdx && n) || (!val && n)) return fail(RCHOL_AMD_EINVAL, "null argument");
+  if (n == 0 || n > kIdxMax) return fail(RCHOL_AMD_ERANGE, "n=%zu outside [1, 2^31)", n);
+  const size_t nnz = rowPtr[n];
+  if (rowPtr[0] != 0) return fail(RCHOL_AMD_EINVAL, "rowPtr[0] must be 0");
+  if (nnz > kIdxMax) return fail(RCHOL_AMD_ERANGE, "nnz(A)=%zu exceeds the int32 index range", nnz);
+  std::vector<int> rp(n + 1), ci(nnz);
+  for (size_t i = 0; i <= n; i++) {
+    if (i && rowPtr[i] < rowPtr[i - 1]) return fail(RCHOL_AMD_EINVAL, "rowPtr not monotone at %zu", i);
+    rp[i] = (int)rowPtr[i];
+  }
+  for (size_t e = 0; e < nnz; e++) {
+    if (colIdx[e] >= n) return fail(RCHOL_AMD_EINVAL, "colIdx[%zu]=%zu out of range", e, colIdx[e]);
+    ci[e] = (int)colIdx[e];
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->drop_graph();
+  c->hasA = false;
+  HIP_TRY(c->arp.upload(rp.data(), n + 1));
+  HIP_TRY(c->aci.upload(ci.data(), nnz));
+  HIP_TRY(c->av.upload(val, nnz));
+  c->n = n;
+  c->nnzA = nnz;
+  c->hasA = true;
+  c->stats.n = n;
+  c->stats.nnz_a = nnz;
+  return RCHOL_AMD_OK;
+}
+
+int rchol_amd_upload_G(rchol_amd_ctx *c, size_t n, const size_t *colPtr, const size_t *rowIdx,
+                       const double *val, int precision) {
+  if (!c || !colPtr || !rowIdx || !val) return fail(RCHOL_AMD_EINVAL, "null argument");
+  if (precision != RCHOL_AMD_FP64 && precision != RCHOL_AMD_FP32)
+    return fail(RCHOL_AMD_EINVAL, "unknown precision %d", precision);
+  if (n == 0 || n > kIdxMax) return fail(RCHOL_AMD_ERANGE, "n=%zu outside [1, 2^31)", n);
+  const size_t nnz = colPtr[n];
+  if (colPtr[0] != 0) return fail(RCHOL_AMD_EINVAL, "colPtr[0] must be 0");
+  if (nnz > kIdxMax) return fail(RCHOL_AMD_ERANGE, "nnz(G)=%zu exceeds the int32 index range", nnz);
+  // Validate lower-triangular columns and find the diagonal of each.
+  std::vector<double> diag(n, 0.0);
+  std::vector<int> offf(n, 0);  // off-diagonals per row of G (forward sweep)
+  for (size_t j = 0; j < n; j++) {
+    if (colPtr[j + 1] < colPtr[j]) return fail(RCHOL_AMD_EINVAL, "colPtr not monotone at %zu", j);
+    bool seen = false;
+    for (size_t e = colPtr[j]; e < colPtr[j + 1]; e++) {
+      size_t i = rowIdx[e];
+      if (i >= n || i < j)
+        return fail(RCHOL_AMD_EINVAL, "G entry (%zu,%zu) is not in the lower triangle", i, j);
+      if (i == j) {
+        if (seen) return fail(RCHOL_AMD_EINVAL, "column %zu holds two diagonal entries", j);
+        seen = true;
+        diag[j] = val[e];
+      } else {
+        offf[i]++;
+      }
+    }
+    if (!seen || diag[j] == 0.0)
+      return fail(RCHOL_AMD_EPIVOT, "G(%zu,%zu) is zero or missing", j, j);
+  }
+  // Dependency levels.  Forward: row i of G waits for every column j < i
+  // with G(i,j) != 0.  Backward (G^T z = y): row i of G^T waits for every
+  // i < j with G(j,i) != 0, i.e. the stored column i.
+  std::vector<int> levf(n, 0), levb(n, 0);
+  int nlf = 0, nlb = 0;
+  for (size_t j = 0; j < n; j++) {
+    const int lj = levf[j] + 1;
+    nlf = std::max(nlf, levf[j] + 1);
+    for (size_t e = colPtr[j]; e < colPtr[j + 1]; e++) {
+      size_t i = rowIdx[e];
+      if (i != j && levf[i] < lj) levf[i] = lj;
+    }
+  }
+  for (size_t jj = n; jj-- > 0;) {
+    int m = 0;
+    for (size_t e = colPtr[jj]; e < colPtr[jj + 1]; e++) {
+      size_t i = rowIdx[e];
+      if (i != jj) m = std::max(m, levb[i] + 1);
+    }
+    levb[jj] = m;
+    nlb = std::max(nlb, m + 1);
+  }
+  HostSweep hf, hb;
+  std::vector<int> slot_of;
+  // forward sweep: CSR of lower G in slot order
+  build_slots(levf, nlf, hf, slot_of);
+  hf.rp.assign(n + 1, 0);
+  for (size_t s = 0; s < n; s++) hf.rp[s + 1] = hf.rp[s] + offf[hf.slot_row[s]];
+  const size_t noff = nnz - n;
+  hf.ci.assign(noff, 0);
+  hf.cv.assign(noff, 0.0);
+  hf.diag.assign(n, 0.0);
+  {
+    std::vector<int> cur(n);
+    for (size_t i = 0; i < n; i++) cur[i] = hf.rp[slot_of[i]];
+    for (size_t j = 0; j < n; j++) {
+      for (size_t e = colPtr[j]; e < colPtr[j + 1]; e++) {
+        size_t i = rowIdx[e];
+        if (i == j) continue;
+        int pos = cur[i]++;
+        hf.ci[pos] = (int)j;
+        hf.cv[pos] = val[e];
+      }
+    }
+    for (size_t s = 0; s < n; s++) hf.diag[s] = diag[hf.slot_row[s]];
+  }
+  // backward sweep: the stored rows (CSR of G^T) in slot order
+  build_slots(levb, nlb, hb, slot_of);
+  hb.rp.assign(n + 1, 0);
+  for (size_t s = 0; s < n; s++) {
+    int row = hb.slot_row[s];
+    hb.rp[s + 1] = hb.rp[s] + (int)(colPtr[row + 1] - colPtr[row] - 1);
+  }
+  hb.ci.assign(noff, 0);
+  hb.cv.assign(noff, 0.0);
+  hb.diag.assign(n, 0.0);
+  for (size_t s = 0; s < n; s++) {
+    int row = hb.slot_row[s];
+    int pos = hb.rp[s];
+    for (size_t e = colPtr[row]; e < colPtr[row + 1]; e++) {
+      size_t j = rowIdx[e];
+      if (j == (size_t)row) continue;
+      hb.ci[pos] = (int)j;
+      hb.cv[pos] = val[e];
+      pos++;
+    }
+    hb.diag[s] = diag[row];
+  }
+
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->drop_graph();
+  c->hasG = false;
+  auto up = [&](Sweep &sw, HostSweep &h) -> int {
+    sw.nslots = (int)n;
+    sw.ntasks = (int)h.task_ptr.size() - 1;
+    sw.nlevels = h.nlevels;
+    sw.nnz_off = noff;
+    HIP_TRY(sw.task_ptr.upload(h.task_ptr.data(), h.task_ptr.size()));
+    HIP_TRY(sw.slot_row.upload(h.slot_row.data(), n));
+    HIP_TRY(sw.rp.upload(h.rp.data(), n + 1));
+    HIP_TRY(sw.ci.upload(h.ci.data(), noff));
+    sw.cv64.reset();
+    sw.diag64.reset();
+    sw.cv32.reset();
+    sw.diag32.reset();
+    if (precision == RCHOL_AMD_FP32) {
+      std::vector<float> f(h.cv.begin(), h.cv.end());
+      HIP_TRY(sw.cv32.upload(f.data(), noff));
+      std::vector<float> d(h.diag.begin(), h.diag.end());
+      HIP_TRY(sw.diag32.upload(d.data(), n));
+    } else {
+      HIP_TRY(sw.cv64.upload(h.cv.data(), noff));
+      HIP_TRY(sw.diag64.upload(h.diag.data(), n));
+    }
+    return RCHOL_AMD_OK;
+  };
+  int rc = up(c->fwd, hf);
+  if (rc) return rc;
+  rc = up(c->bwd, hb);
+  if (rc) return rc;
+  c->nG = n;
+  c->nnzG = nnz;
+  c->gprec = precision;
+  rc = c->size_trsv_grid();
+  if (rc) return rc;
+  c->hasG = true;
+  c->stats.nnz_g = nnz;
+  c->stats.fwd_levels = nlf;
+  c->stats.bwd_levels = nlb;
+  c->stats.fwd_tasks = c->fwd.ntasks;
+  c->stats.bwd_tasks = c->bwd.ntasks;
+  c->stats.g_precision = precision;
+  return RCHOL_AMD_OK;
+}
+
+int rchol_amd_set_option(rchol_amd_ctx *c, int key, double value) {
+  if (!c) return fail(RCHOL_AMD_EINVAL, "null context");
+  int v = (int)value;
+  switch (key) {
+    case RCHOL_AMD_OPT_GRAPH_ITERS:
+      if (v < 0 || v > 1024) return fail(RCHOL_AMD_EINVAL, "graph iterations %d out of [0,1024]", v);
+      c->graph_iters = v;
+      break;
+    case RCHOL_AMD_OPT_RESID_REPLACE:
+      if (v < 0) return fail(RCHOL_AMD_EINVAL, "negative replacement period");
+      c->resid_period = v;
+      break;
+    case RCHOL_AMD_OPT_SPMV_LANES:
+      if (v != 4 && v != 8 && v != 16 && v != 32) return fail(RCHOL_AMD_EINVAL, "lanes must be 4/8/16/32");
+      c->spmv_lanes = v;
+      break;
+    case RCHOL_AMD_OPT_TRSV_LANES:
+      if (v != 4 && v != 8 && v != 16 && v != 32) return fail(RCHOL_AMD_EINVAL, "lanes must be 4/8/16/32");
+      c->trsv_lanes = v;
+      break;
+    default:
+      return fail(RCHOL_AMD_EINVAL, "unknown option key %d", key);
+  }
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  c->drop_graph();
+  if (key == RCHOL_AMD_OPT_TRSV_LANES && c->hasG) return c->size_trsv_grid();
+  return RCHOL_AMD_OK;
+}
+
+int rchol_amd_get_stats(rchol_amd_ctx *c, rchol_amd_stats *out) {
+  if (!c || !out) return fail(RCHOL_AMD_EINVAL, "null argument");
+  *out = c->stats;
+  return RCHOL_AMD_OK;
+}
+
+int rchol_amd_precond_device(rchol_amd_ctx *c, const double *d_r, double *d_z) {
+  if (!c || !d_r || !d_z) return fail(RCHOL_AMD_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(c->device));
+  return c->precond(d_r, d_z);
+}
+
+int rchol_amd_precond(rchol_amd_ctx *c, const double *r, double *z) {
+  if (!c || !r || !z) return fail(RCHOL_AMD_EINVAL, "null argument");
+  if (!c->hasG) return fail(RCHOL_AMD_ESTATE, "G must be uploaded before precond");
+  HIP_TRY(hipSetDevice(c->device));
+  const size_t n = c->nG;
+  DevBuf<double> dr, dz;
+  HIP_TRY(dr.upload(r, n));
+  HIP_TRY(dz.alloc(n));
+  int rc = c->precond(dr.p, dz.p);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(z, dz.p, n * sizeof(double), hipMemcpyDeviceToHost));
+  return RCHOL_AMD_OK;
+}
+
+int rchol_amd_spmv(rchol_amd_ctx *c, const double *xh, double *yh) {
+  if (!c || !xh || !yh) return fail(RCHOL_AMD_EINVAL, "null argument");
+  if (!c->hasA) return fail(RCHOL_AMD_ESTATE, "A must be uploaded before spmv");
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = c->ensure_work();
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(c->tmp64a.p, xh, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  launch_spmv_res(c, c->tmp64a.p, nullptr, c->q.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(yh, c->q.p, c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RCHOL_AMD_OK;
+}
+
+int rchol_amd_pcg_device(rchol_amd_ctx *c, const double *d_b, double tol, int maxit, double *d_x,
+                         int x_in_is_initial, double *relres, int *itr) {
+  if (!c || !d_b || !d_x || !relres || !itr) return fail(RCHOL_AMD_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(c->device));
+  return c->pcg(d_b, tol, maxit, d_x, x_in_is_initial, relres, itr);
+}
+
+int rchol_amd_pcg(rchol_amd_ctx *c, const double *bh, double tol, int maxit, double *xh,
+                  int x_in_is_initial, double *relres, int *itr) {
+  if (!c || !bh || !xh || !relres || !itr) return fail(RCHOL_AMD_EINVAL, "null argument");
+  if (!c->hasA) return fail(RCHOL_AMD_ESTATE, "A must be uploaded before pcg");
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = c->ensure_work();
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(c->b.p, bh, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (x_in_is_initial)
+    HIP_TRY(hipMemcpyAsync(c->x.p, xh, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  rc = c->pcg(c->b.p, tol, maxit, c->x.p, x_in_is_initial, relres, itr);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(xh, c->x.p, c->n * sizeof(double), hipMemcpyDeviceToHost));
+  return RCHOL_AMD_OK;
+}
+
+int rchol_amd_pcg_csr(size_t n, const size_t *aRowPtr, const size_t *aColIdx, const double *aVal,
+                      const double *b, double tol, int maxit, const size_t *gColPtr,
+                      const size_t *gRowIdx, const double *gVal, double *x, int x_in_is_initial,
+                      double *relres, int *itr) {
+  rchol_amd_ctx *c = nullptr;
+  int rc = rchol_amd_create(0, &c);
+  if (rc) return rc;
+  rc = rchol_amd_upload_A(c, n, aRowPtr, aColIdx, aVal);
+  if (!rc) rc = rchol_amd_upload_G(c, n, gColPtr, gRowIdx, gVal, RCHOL_AMD_FP64);
+  if (!rc) rc = rchol_amd_pcg(c, b, tol, maxit, x, x_in_is_initial, relres, itr);
+  std::string keep = g_err;
+  rchol_amd_destroy(c);
+  g_err = keep;
+  return rc;
+}
+
+}  // extern "C"

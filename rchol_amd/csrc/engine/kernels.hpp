@@ -1,0 +1,228 @@
+// Solve kernels for gfx950 (wave64).  All are HBM-bandwidth kernels: no
+// dense contraction exists on this path, so none uses MFMA.
+//
+// Index arrays are int32 (the host checks the range), values FP64 or FP32.
+// A row group is L consecutive lanes; a 64-lane wave holds 64/L row groups.
+#pragma once
+#include "device.hpp"
+
+namespace rchol_amd {
+
+// ---------------------------------------------------------------------------
+// Sparse triangular sweep, pull form, persistent and level-scheduled.
+//
+// Rows are stored in "slot" order: sorted by dependency level, so a slot
+// range of one level holds mutually independent rows.  The host cuts each
+// level into tasks of at most 64 slots; task t belongs to wave t % nwaves
+// (static round robin, no atomics).  A row reads x[j] of its dependencies;
+// unpublished entries still hold the sentinel and are polled until the
+// producing wave stores the value (agent-scope atomic store).  Deadlock
+// freedom: every launched wave is resident (the grid is sized from the
+// occupancy query), each wave walks its tasks in increasing order and a task
+// only depends on tasks of lower levels, so the smallest unfinished task
+// always has all inputs and is being worked on.
+//
+//   x[row] = (rhs[row] - sum_e val[e] * x[col[e]]) / diag[slot]
+//
+// FUSE == 1: also store the sentinel into aux[row] (prepares the output of
+//            the following sweep, which must start from sentinels).
+// FUSE == 2: accumulate dotv[row] * x[row] and reduce it on the grid into
+//            sc->rz; the last block forms beta (pcg.cpp:93-95).
+// ---------------------------------------------------------------------------
+template <typename VT, typename XT, typename BT, int L, int FUSE>
+__global__ __launch_bounds__(256) void k_sptrsv(
+    int ntasks, const int *__restrict__ task_ptr, const int *__restrict__ slot_row,
+    const int *__restrict__ rp, const int *__restrict__ ci, const VT *__restrict__ cv,
+    const VT *__restrict__ diag, const BT *__restrict__ rhs, XT *x, XT *__restrict__ aux,
+    const double *__restrict__ dotv, double *__restrict__ partials, DevScalars *sc,
+    const int *stop) {
+  if (*stop) return;
+  constexpr int R = 64 / L;
+  const int lane = threadIdx.x & 63;
+  const int grp = lane / L, sub = lane % L;
+  const int nwaves = gridDim.x * (blockDim.x >> 6);
+  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  int *err = &sc->err;
+  double dacc = 0.0;
+  for (int t = wave; t < ntasks; t += nwaves) {
+    const int s0 = task_ptr[t], s1 = task_ptr[t + 1];
+    for (int s = s0 + grp; s < s1; s += R) {
+      const int row = slot_row[s];
+      const int e0 = rp[s], e1 = rp[s + 1];
+      XT acc = 0;
+      for (int e = e0 + sub; e < e1; e += L) {
+        const int j = ci[e];
+        const XT xj = wait_ready<XT>(x + j, err);
+        acc += (XT)cv[e] * xj;
+      }
+      acc = group_sum<L>(acc);
+      if (sub == 0) {
+        const XT v = ((XT)rhs[row] - acc) / (XT)diag[s];
+        Sent<XT>::store_agent(x + row, v);
+        if (FUSE == 1) aux[row] = Sent<XT>::value();
+        if (FUSE == 2) dacc += dotv[row] * (double)v;
+      }
+    }
+  }
+  if (FUSE == 2) {
+    double v[1] = {dacc}, tot[1];
+    if (grid_reduce<1>(v, partials, &sc->ctr[CTR_BWD], tot)) {
+      const double rz = tot[0];
+      sc->rz = rz;
+      sc->beta = (sc->k > 0) ? rz / sc->rz_old : 0.0;
+      sc->rz_old = rz;
+    }
+  }
+}
+
+// Fill a vector with the sentinel (before a sweep) -- a kernel, not a memset,
+// so that it can sit in a captured graph.
+template <typename T>
+__global__ void k_fill_sentinel(int n, T *x) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    x[i] = Sent<T>::value();
+}
+
+__global__ void k_widen(int n, const float *__restrict__ a, double *__restrict__ b) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    b[i] = (double)a[i];
+}
+
+// ---------------------------------------------------------------------------
+// PCG vector stages.  Grid-stride, a fixed grid so that the last-block
+// reductions see a bounded number of partials.
+// ---------------------------------------------------------------------------
+
+// r = b, x = 0 (or keep), y = sentinel, ||b||^2; sets the loop test of
+// pcg.cpp:82 for k = 0.
+template <typename YT>
+__global__ __launch_bounds__(256) void k_init(int n, const double *__restrict__ b, double *r,
+                                              double *x, int keep_x, YT *y, double *partials,
+                                              DevScalars *sc) {
+  double acc = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double bi = b[i];
+    r[i] = bi;
+    if (!keep_x) x[i] = 0.0;
+    y[i] = Sent<YT>::value();
+    acc += bi * bi;
+  }
+  double v[1] = {acc}, tot[1];
+  if (grid_reduce<1>(v, partials, &sc->ctr[CTR_INIT], tot)) {
+    sc->bb = tot[0];
+    sc->rr = tot[0];
+    sc->k = 0;
+    sc->err = 0;
+    sc->beta = 0.0;
+    sc->done = !(sqrt(tot[0]) > sqrt(tot[0]) * sc->tol && 0 < sc->maxit);
+  }
+}
+
+// p_new = z + beta p_old (p = z when k == 0, pcg.cpp:87-96), q = A p_new
+// (pcg.cpp:100), and the two dots p.r, p.q (pcg.cpp:101-103) -> alpha.
+// A row's neighbours p_new[j] are recomputed from z and p_old (bitwise the
+// same value), so p_new need not be complete before q is formed.
+template <typename ZT, int L>
+__global__ __launch_bounds__(256) void k_spmv_p(int n, const int *__restrict__ rp,
+                                                const int *__restrict__ ci,
+                                                const double *__restrict__ av,
+                                                const ZT *__restrict__ z,
+                                                const double *__restrict__ p_old,
+                                                double *__restrict__ p_new, double *__restrict__ q,
+                                                const double *__restrict__ r, double *partials,
+                                                DevScalars *sc) {
+  if (sc->done) return;
+  constexpr int R = 64 / L;
+  const double beta = sc->beta;
+  const bool first = (sc->k == 0);
+  const int lane = threadIdx.x & 63, grp = lane / L, sub = lane % L;
+  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int nw = gridDim.x * (blockDim.x >> 6);
+  double apr = 0.0, apq = 0.0;
+  for (int row = gw * R + grp; row < n; row += nw * R) {
+    const int e0 = rp[row], e1 = rp[row + 1];
+    double acc = 0.0;
+    for (int e = e0 + sub; e < e1; e += L) {
+      const int j = ci[e];
+      const double pj = first ? (double)z[j] : beta * p_old[j] + (double)z[j];
+      acc += av[e] * pj;
+    }
+    acc = group_sum<L>(acc);
+    if (sub == 0) {
+      const double pi = first ? (double)z[row] : beta * p_old[row] + (double)z[row];
+      p_new[row] = pi;
+      q[row] = acc;
+      apr += pi * r[row];
+      apq += pi * acc;
+    }
+  }
+  double v[2] = {apr, apq}, tot[2];
+  if (grid_reduce<2>(v, partials, &sc->ctr[CTR_SPMVP], tot)) {
+    sc->pr = tot[0];
+    sc->pq = tot[1];
+    sc->alpha = tot[0] / tot[1];
+  }
+}
+
+// x += alpha p, r -= alpha q (pcg.cpp:104,108), ||r||^2, y = sentinel for
+// the next forward sweep; the last block advances k and evaluates the loop
+// test of pcg.cpp:82.
+template <typename YT>
+__global__ __launch_bounds__(256) void k_update(int n, const double *__restrict__ p,
+                                                const double *__restrict__ q, double *x, double *r,
+                                                YT *y, double *partials, DevScalars *sc) {
+  if (sc->done) return;
+  const double alpha = sc->alpha;
+  double acc = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    y[i] = Sent<YT>::value();
+    acc += ri * ri;
+  }
+  double v[1] = {acc}, tot[1];
+  if (grid_reduce<1>(v, partials, &sc->ctr[CTR_UPD], tot)) {
+    const int k = sc->k + 1;
+    sc->k = k;
+    sc->rr = tot[0];
+    sc->done = !(sqrt(tot[0]) > sqrt(sc->bb) * sc->tol && k < sc->maxit);
+  }
+}
+
+// q = A x - b and ||q||^2 -> relres (pcg.cpp:116-118).  With bvec == null it
+// is a plain y = A x.
+template <int L>
+__global__ __launch_bounds__(256) void k_spmv_res(int n, const int *__restrict__ rp,
+                                                  const int *__restrict__ ci,
+                                                  const double *__restrict__ av,
+                                                  const double *__restrict__ x,
+                                                  const double *__restrict__ bvec,
+                                                  double *__restrict__ y, double *partials,
+                                                  DevScalars *sc) {
+  constexpr int R = 64 / L;
+  const int lane = threadIdx.x & 63, grp = lane / L, sub = lane % L;
+  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int nw = gridDim.x * (blockDim.x >> 6);
+  double aqq = 0.0;
+  for (int row = gw * R + grp; row < n; row += nw * R) {
+    const int e0 = rp[row], e1 = rp[row + 1];
+    double acc = 0.0;
+    for (int e = e0 + sub; e < e1; e += L) acc += av[e] * x[ci[e]];
+    acc = group_sum<L>(acc);
+    if (sub == 0) {
+      if (bvec) acc -= bvec[row];
+      y[row] = acc;
+      aqq += acc * acc;
+    }
+  }
+  if (bvec) {
+    double v[1] = {aqq}, tot[1];
+    if (grid_reduce<1>(v, partials, &sc->ctr[CTR_RES], tot)) {
+      sc->qq = tot[0];
+      sc->relres = sqrt(tot[0]) / sqrt(sc->bb);
+    }
+  }
+}
+
+}  // namespace rchol_amd

@@ -1,0 +1,193 @@
+"""Device-resident PCG solver over the C ABI of librchol_amd.so.
+
+Python face of the hot path of the reference (c++/util/pcg.cpp and
+python/rchol/util.py:32-61).  A ``Solver`` owns one engine context: A and the
+factor G are uploaded once and stay resident in HBM across solves.
+"""
+import ctypes
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+
+FP64 = 0
+FP32 = 1
+
+OPT_GRAPH_ITERS = 1
+OPT_RESID_REPLACE = 2
+OPT_SPMV_LANES = 3
+OPT_TRSV_LANES = 4
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_size_t),
+        ("nnz_a", ctypes.c_size_t),
+        ("nnz_g", ctypes.c_size_t),
+        ("fwd_levels", ctypes.c_int),
+        ("bwd_levels", ctypes.c_int),
+        ("fwd_tasks", ctypes.c_int),
+        ("bwd_tasks", ctypes.c_int),
+        ("trsv_waves", ctypes.c_int),
+        ("g_precision", ctypes.c_int),
+        ("last_iters", ctypes.c_int),
+        ("last_solve_ms", ctypes.c_double),
+        ("last_relres", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+def _u64(a):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    return a, a.ctypes.data_as(_lib.u64p)
+
+
+def _f64(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_lib.f64p)
+
+
+def device_count():
+    return _lib.engine().rchol_amd_device_count()
+
+
+def stored_from_lower(G):
+    """Stored factor arrays (CSC of lower G == CSR of G^T) from a scipy G."""
+    Gc = sp.csc_matrix(G)
+    Gc.sort_indices()
+    return Gc.indptr, Gc.indices, Gc.data
+
+
+class Solver:
+    """One engine context: A and G resident on one GPU.
+
+    A: scipy sparse (n x n), used as CSR.
+    G: either the stored triple (colPtr, rowIdx, val) exactly as the C++
+       rchol() fills SparseCSR G, or a scipy lower-triangular matrix.
+    precision: FP64 or FP32 for the factor and the two triangular sweeps.
+    """
+
+    def __init__(self, A=None, G=None, precision=FP64, device=0, graph_iters=None,
+                 spmv_lanes=None, trsv_lanes=None):
+        self._lib = _lib.engine()
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.rchol_amd_create(int(device), ctypes.byref(h)))
+        self._h = h
+        self.n = None
+        if graph_iters is not None:
+            self.set_option(OPT_GRAPH_ITERS, graph_iters)
+        if spmv_lanes is not None:
+            self.set_option(OPT_SPMV_LANES, spmv_lanes)
+        if trsv_lanes is not None:
+            self.set_option(OPT_TRSV_LANES, trsv_lanes)
+        if A is not None:
+            self.upload_A(A)
+        if G is not None:
+            self.upload_G(G, precision)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.rchol_amd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_option(self, key, value):
+        _lib.check(self._lib.rchol_amd_set_option(self._h, int(key), float(value)))
+
+    def upload_A(self, A):
+        A = sp.csr_matrix(A)
+        A.sort_indices()
+        n = A.shape[0]
+        rp, prp = _u64(A.indptr)
+        ci, pci = _u64(A.indices)
+        v, pv = _f64(A.data)
+        _lib.check(self._lib.rchol_amd_upload_A(self._h, n, prp, pci, pv))
+        self.n = n
+
+    def upload_G(self, G, precision=FP64):
+        if isinstance(G, tuple):
+            cp, ri, gv = G
+        else:
+            cp, ri, gv = stored_from_lower(G)
+        cp, pcp = _u64(cp)
+        ri, pri = _u64(ri)
+        gv, pgv = _f64(gv)
+        n = cp.size - 1
+        _lib.check(self._lib.rchol_amd_upload_G(self._h, n, pcp, pri, pgv, int(precision)))
+        if self.n is None:
+            self.n = n
+
+    def precond(self, r):
+        """z = G^-T G^-1 r (pcg.cpp:141-159)."""
+        r, pr = _f64(r)
+        z = np.empty_like(r)
+        _lib.check(self._lib.rchol_amd_precond(self._h, pr, z.ctypes.data_as(_lib.f64p)))
+        return z
+
+    def spmv(self, x):
+        x, px = _f64(x)
+        y = np.empty_like(x)
+        _lib.check(self._lib.rchol_amd_spmv(self._h, px, y.ctypes.data_as(_lib.f64p)))
+        return y
+
+    def pcg(self, b, tol, maxit, x0=None):
+        """PCG of pcg.cpp:57-127 on host arrays -> (x, relres, itr)."""
+        b, pb = _f64(b)
+        if x0 is None:
+            x = np.zeros_like(b)
+            keep = 0
+        else:
+            x = np.array(x0, dtype=np.float64, copy=True)
+            keep = 1
+        relres = ctypes.c_double()
+        itr = ctypes.c_int()
+        _lib.check(self._lib.rchol_amd_pcg(self._h, pb, float(tol), int(maxit),
+                                           x.ctypes.data_as(_lib.f64p), keep,
+                                           ctypes.byref(relres), ctypes.byref(itr)))
+        return x, relres.value, itr.value
+
+    def pcg_device(self, b_ptr, x_ptr, tol, maxit, x_is_initial=False):
+        """PCG on device pointers (e.g. torch tensors' data_ptr())."""
+        relres = ctypes.c_double()
+        itr = ctypes.c_int()
+        _lib.check(self._lib.rchol_amd_pcg_device(self._h, ctypes.c_void_p(b_ptr), float(tol),
+                                                  int(maxit), ctypes.c_void_p(x_ptr),
+                                                  int(bool(x_is_initial)),
+                                                  ctypes.byref(relres), ctypes.byref(itr)))
+        return relres.value, itr.value
+
+    def precond_device(self, r_ptr, z_ptr):
+        _lib.check(self._lib.rchol_amd_precond_device(self._h, ctypes.c_void_p(r_ptr),
+                                                      ctypes.c_void_p(z_ptr)))
+
+    def stats(self):
+        s = Stats()
+        _lib.check(self._lib.rchol_amd_get_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+
+def pcg(A, b, epsilon, maxitr, G, Gt=None):
+    """Drop-in for python/rchol/util.py:32-61: pcg(A, b, epsilon, maxitr, G, Gt).
+
+    G is the lower-triangular factor (scipy), Gt its transpose (accepted for
+    signature compatibility; the engine derives both sweeps from G).
+    Returns (x, relres, niters) with relres the true residual
+    ||A x - b|| / ||b|| (util.py:56).  The preconditioner applied is
+    G^-T G^-1 as in c++/util/pcg.cpp:141-159.
+    """
+    with Solver(A, G) as s:
+        return s.pcg(b, epsilon, maxitr)

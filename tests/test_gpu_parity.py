@@ -1,0 +1,160 @@
+"""GPU parity of the HIP path against the oracle (CPU restatement of
+c++/util/pcg.cpp), through the C ABI.
+
+Bars: FP64 preconditioner application within 1e-12
+relative (2-norm) of the oracle; PCG iteration count to 1e-10 within +-1 of
+the oracle; SpMV within 1e-13 relative.  The FP32 factor path is held to
+1e-5 relative on the application and to an FP64-verified 1e-10 solve.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import rchol_amd
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+FIXTURES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLD, "*.npz")))
+
+PRECOND_RTOL = 1e-12
+SPMV_RTOL = 1e-13
+
+
+def load(name):
+    d = np.load(os.path.join(GOLD, name + ".npz"))
+    n = d["a_indptr"].size - 1
+    A = sp.csr_matrix((d["a_data"], d["a_indices"], d["a_indptr"]), shape=(n, n))
+    return A, (d["g_colptr"], d["g_rowidx"], d["g_val"]), d
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if rchol_amd.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_precond_golden(name):
+    A, Gs, d = load(name)
+    print(f"case precond_golden {name} n={A.shape[0]}", flush=True)
+    with rchol_amd.Solver(A, Gs) as s:
+        z = s.precond(d["b"])
+        assert rel(z, d["z"]) <= PRECOND_RTOL
+        # repeated applications are identical (deterministic sweeps)
+        z2 = s.precond(d["b"])
+        np.testing.assert_array_equal(z, z2)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_spmv_golden(name):
+    A, Gs, d = load(name)
+    print(f"case spmv_golden {name}", flush=True)
+    with rchol_amd.Solver(A, Gs) as s:
+        y = s.spmv(d["b"])
+        assert rel(y, d["Ab"]) <= SPMV_RTOL
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+@pytest.mark.parametrize("graph_iters", [0, 16])
+def test_pcg_golden(name, graph_iters):
+    A, Gs, d = load(name)
+    print(f"case pcg_golden {name} graph_iters={graph_iters}", flush=True)
+    with rchol_amd.Solver(A, Gs, graph_iters=graph_iters) as s:
+        x, relres, itr = s.pcg(d["b"], float(d["tol"]), 5000)
+    assert abs(itr - int(d["itr"])) <= 1
+    assert relres <= float(d["tol"]) * 10
+    assert rel(x, d["x"]) <= 1e-8
+    true = np.linalg.norm(A @ x - d["b"]) / np.linalg.norm(d["b"])
+    assert relres == pytest.approx(true, rel=1e-6)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_precond_fp32_golden(name):
+    A, Gs, d = load(name)
+    print(f"case precond_fp32 {name}", flush=True)
+    with rchol_amd.Solver(A, Gs, precision=rchol_amd.FP32) as s:
+        z = s.precond(d["b"])
+    assert rel(z, d["z"]) <= 1e-5
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_pcg_fp32_factor_reaches_fp64_tolerance(name):
+    A, Gs, d = load(name)
+    print(f"case pcg_fp32 {name}", flush=True)
+    with rchol_amd.Solver(A, Gs, precision=rchol_amd.FP32) as s:
+        x, relres, itr = s.pcg(d["b"], 1e-10, 5000)
+    true = np.linalg.norm(A @ x - d["b"]) / np.linalg.norm(d["b"])
+    assert true <= 1e-9
+    assert itr <= int(d["itr"]) + 5
+
+
+def test_pcg_reference_semantics_edges():
+    A, Gs, d = load("lap3d_4_seq")
+    b = d["b"]
+    print("case pcg_edges", flush=True)
+    with rchol_amd.Solver(A, Gs) as s:
+        # maxit cut-off is silent (pcg.cpp:82)
+        x, relres, itr = s.pcg(b, 1e-14, 3)
+        xo, ro, io = O.pcg(A, b, 1e-14, 3, Gs)
+        assert itr == io == 3
+        assert rel(x, xo) <= 1e-11
+        # maxit = 0: no iteration, x stays zero, relres = 1
+        x, relres, itr = s.pcg(b, 1e-10, 0)
+        assert itr == 0 and np.all(x == 0) and relres == pytest.approx(1.0)
+        # pre-sized x is an accumulator while r starts at b (pcg.cpp:67-71)
+        x0 = np.full(b.size, 0.25)
+        x1, _, i1 = s.pcg(b, 1e-10, 7, x0=x0)
+        xo1, _, _ = O.pcg(A, b, 1e-10, 7, Gs, x0=x0)
+        assert rel(x1, xo1) <= 1e-10
+        # repeated solves on the same context reuse the residency
+        for _ in range(3):
+            x, relres, itr = s.pcg(b, 1e-10, 500)
+            assert abs(itr - int(d["itr"])) <= 1
+
+
+def test_exact_factor_one_iteration():
+    n = 64
+    e = np.ones(n)
+    A = sp.diags([-e[:-1], 2 * e, -e[:-1]], [-1, 0, 1], format="csr")
+    Gc = sp.csc_matrix(np.linalg.cholesky(A.toarray()))
+    b = np.random.default_rng(0).uniform(-1, 1, n)
+    print("case exact_factor", flush=True)
+    with rchol_amd.Solver(A, (Gc.indptr, Gc.indices, Gc.data)) as s:
+        x, relres, itr = s.pcg(b, 1e-10, 10)
+    assert itr == 1
+    assert rel(x, np.linalg.solve(A.toarray(), b)) <= 1e-12
+
+
+def test_diagonal_factor_and_single_row():
+    print("case diagonal_factor", flush=True)
+    A = sp.csr_matrix(np.array([[4.0]]))
+    with rchol_amd.Solver(A, (np.array([0, 1]), np.array([0]), np.array([2.0]))) as s:
+        x, relres, itr = s.pcg(np.array([8.0]), 1e-10, 10)
+        assert itr == 1 and x[0] == pytest.approx(2.0)
+        z = s.precond(np.array([8.0]))
+        assert z[0] == pytest.approx(2.0)
+
+
+def test_upload_rejects_bad_factor():
+    A, Gs, d = load("lap3d_4_seq")
+    print("case bad_factor", flush=True)
+    with rchol_amd.Solver(A) as s:
+        cp, ri, v = (np.array(a, copy=True) for a in Gs)
+        v[cp[3]] = 0.0  # zero pivot of column 3
+        with pytest.raises(RuntimeError, match="error -5"):
+            s.upload_G((cp, ri, v))
+        cp, ri, v = (np.array(a, copy=True) for a in Gs)
+        ri[cp[5]], ri[cp[5] + 1] = ri[cp[5] + 1], ri[cp[5]]  # diagonal not first is fine,
+        s.upload_G((cp, ri, v))                               # but must still be lower
+        bad = (np.array([0, 1, 2]), np.array([1, 1]), np.array([1.0, 1.0]))
+        with pytest.raises(RuntimeError):
+            s.upload_G(bad)  # column 0 misses its diagonal
