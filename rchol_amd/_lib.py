@@ -42,7 +42,35 @@ ENGINE_SYMBOLS = {
     "rchol_amd_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
 }
 
-FACTOR_SYMBOLS = {}
+size_tp = ctypes.POINTER(ctypes.c_size_t)
+pu64p = ctypes.POINTER(u64p)
+pf64p = ctypes.POINTER(f64p)
+
+
+class FactorOut(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_size_t), ("colPtr", u64p), ("rowIdx", u64p), ("val", f64p),
+                ("perm", u64p), ("seconds", ctypes.c_double)]
+
+
+# Every C-ABI symbol of include/rchol_factor.h with its signature.
+FACTOR_SYMBOLS = {
+    "rchol_factor_last_error": (ctypes.c_char_p, []),
+    "rchol_factor_free": (None, [ctypes.c_void_p]),
+    "rchol_factor": (ctypes.c_int, [ctypes.c_size_t, u64p, u64p, f64p, u64p, u64p,
+                                    ctypes.c_size_t, ctypes.c_int, ctypes.c_uint,
+                                    ctypes.POINTER(FactorOut)]),
+    "rchol_nd_grid": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     u64p, u64p]),
+    "rchol_nd_graph": (ctypes.c_int, [ctypes.c_size_t, u64p, u64p, ctypes.c_int, u64p, u64p]),
+    "rchol_laplace_3d": (ctypes.c_int, [ctypes.c_int, size_tp, pu64p, pu64p, pf64p]),
+    "rchol_varcoef_3d": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong,
+                                        size_tp, pu64p, pu64p, pf64p]),
+    "rchol_grid_laplacian": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, f64p,
+                                            ctypes.c_double, size_tp, pu64p, pu64p, pf64p]),
+    "rchol_grid_edges": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, size_tp,
+                                        pu64p, pu64p]),
+}
 
 _engine = None
 _factor = None

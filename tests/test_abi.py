@@ -26,6 +26,15 @@ def test_engine_exports_every_declared_symbol():
     assert set(names) == set(_lib.ENGINE_SYMBOLS)
 
 
+def test_factor_library_exports_every_declared_symbol():
+    lib = _lib.factor()
+    names = declared("rchol_factor.h")
+    assert len(names) >= 8
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(_lib.FACTOR_SYMBOLS)
+
+
 def test_engine_reports_errors_without_exceptions():
     lib = _lib.engine()
     if lib.rchol_amd_device_count() > 0:
