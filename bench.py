@@ -1,0 +1,232 @@
+#!/usr/bin/env python3
+"""PCG time-to-1e-10 on the 256^3 variable-coefficient Poisson problem,
+mixed precision (FP32 factor sweeps) vs FP64, on one MI355X.
+
+A step is one complete PCG solve (pcg.cpp:57-127 semantics) from x0 = 0 to
+a relative residual of 1e-10 of the headline (mixed) variant, with A, G and
+b already resident in HBM.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--side 256] [--leaves 128]
+
+The CPU baseline is the oracle (C restatement of c++/util/pcg.cpp) timed on
+a bounded sample of the same system on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_bytes(n, nnz_a, nnz_g, fp32):
+    """Algorithmic (compulsory) HBM bytes per launch of the four loop kernels:
+    each array is touched once; gathers of x are cache hits."""
+    sv = 4 if fp32 else 8   # factor value bytes
+    sx = 4 if fp32 else 8   # sweep vector bytes (y, z)
+    fwd = nnz_g * (sv + 4) + 4 * (n + 1) + 8 * n + sx * n
+    bwd = nnz_g * (sv + 4) + 4 * (n + 1) + sx * n + sx * n + 8 * n
+    spmv = 12 * nnz_a + 4 * (n + 1) + sx * n + 8 * n + 8 * n + 16 * n
+    upd = 32 * n + 16 * n + sx * n
+    return [fwd, bwd, spmv, upd]
+
+
+KERNEL_NAMES = ["sptrsv_fwd", "sptrsv_bwd", "spmv_p", "update"]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--side", type=int, default=256)
+    ap.add_argument("--leaves", type=int, default=128)
+    ap.add_argument("--order", default="mis", choices=["mis", "static"])
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--maxit", type=int, default=2000)
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ref", action="store_true", help="skip the 1e-14 FP64 reference solve")
+    ap.add_argument("--graph-iters", type=int, default=16)
+    ap.add_argument("--trsv-lanes", type=int, default=8)
+    ap.add_argument("--spmv-lanes", type=int, default=8)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is design-only)", file=sys.stderr)
+        return 2
+
+    import rchol_amd
+    from rchol_amd import factor as F
+
+    if rchol_amd.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        return 1
+
+    s = args.side
+    depth = int(np.log2(args.leaves)) + 1
+    seeds = {"A": 20261202, "G": 1, "b": 7}
+    log(f"generating {s}^3 variable-coefficient SDDM")
+    A = F.varcoef_3d(s, s, s, 1.5, 1e-3, seeds["A"])
+    n = A.shape[0]
+    log(f"n={n} nnz={A.nnz}; nested dissection, {args.leaves} leaves")
+    perm, blocks = F.nd_grid(s, s, s, depth)
+    log(f"rchol factorization ({args.order} order inside ND blocks)")
+    t0 = time.perf_counter()
+    Gs, po, t_elim = F.factor(A, perm, blocks, args.order, seeds["G"])
+    t_factor = time.perf_counter() - t0
+    nnz_g = int(Gs[1].size)
+    log(f"factor done in {t_factor:.1f}s (elimination {t_elim:.1f}s), nnz(G)={nnz_g}, "
+        f"fill={2 * nnz_g / A.nnz:.3f}")
+    P = po.astype(np.int64)
+    Ap = A[P][:, P].tocsr()
+    Ap.sort_indices()
+    del A
+    b = np.random.default_rng(seeds["b"]).uniform(-1.0, 1.0, n)
+    bp = b[P]
+
+    log("uploading (FP64 and FP32 factor contexts)")
+    s64 = rchol_amd.Solver(Ap, Gs, rchol_amd.FP64, graph_iters=args.graph_iters,
+                           spmv_lanes=args.spmv_lanes, trsv_lanes=args.trsv_lanes)
+    s32 = rchol_amd.Solver(Ap, Gs, rchol_amd.FP32, graph_iters=args.graph_iters,
+                           spmv_lanes=args.spmv_lanes, trsv_lanes=args.trsv_lanes)
+    st64 = s64.stats()
+    log(f"sweep levels fwd={st64['fwd_levels']} bwd={st64['bwd_levels']} "
+        f"waves={st64['trsv_waves']}")
+
+    def run(solver, k):
+        times, dev = [], []
+        out = None
+        for _ in range(k):
+            t = time.perf_counter()
+            out = solver.pcg(bp, args.tol, args.maxit)
+            times.append(time.perf_counter() - t)
+            dev.append(solver.stats()["last_solve_ms"] / 1e3)
+        return out, times, dev
+
+    log(f"warmup {args.warmup}")
+    run(s32, args.warmup)
+    run(s64, args.warmup)
+    log(f"timed {args.steps} mixed solves")
+    (x32, rr32, it32), t32, d32 = run(s32, args.steps)
+    log(f"mixed: {it32} iterations, {np.mean(t32) * 1e3:.1f} ms/solve, relres {rr32:.2e}")
+    (x64, rr64, it64), t64, d64 = run(s64, args.steps)
+    log(f"fp64: {it64} iterations, {np.mean(t64) * 1e3:.1f} ms/solve, relres {rr64:.2e}")
+
+    # per-kernel device time (HIP events on the engine's stream) over one
+    # profiled solve of each variant
+    prof = {}
+    for name, solver, fp32 in (("mixed", s32, True), ("fp64", s64, False)):
+        solver.set_option(rchol_amd.solver.OPT_PROFILE, 1)
+        solver.pcg(bp, args.tol, args.maxit)
+        st = solver.stats()
+        solver.set_option(rchol_amd.solver.OPT_PROFILE, 0)
+        iters = max(st["kernel_iters"], 1)
+        avg_ms = [m / iters for m in st["kernel_ms"]]
+        bts = kernel_bytes(n, Ap.nnz, nnz_g, fp32)
+        prof[name] = {
+            KERNEL_NAMES[i]: {"avg_ms": avg_ms[i], "bytes": bts[i],
+                              "gbs": bts[i] / (avg_ms[i] * 1e-3) / 1e9 if avg_ms[i] > 0 else None}
+            for i in range(4)}
+        prof[name]["iters"] = st["kernel_iters"]
+    log("per-kernel: " + json.dumps({k: {kk: round(vv["avg_ms"], 4) for kk, vv in v.items()
+                                         if isinstance(vv, dict)} for k, v in prof.items()}))
+
+    err = {}
+    if not args.no_ref:
+        log("FP64 reference solve to 1e-14")
+        xref, rref, iref = s64.pcg(bp, 1e-14, 20000)
+        nref = np.linalg.norm(xref)
+        err = {"ref_tol": 1e-14, "ref_iters": iref, "ref_relres": rref,
+               "mixed_err_vs_ref": float(np.linalg.norm(x32 - xref) / nref),
+               "fp64_err_vs_ref": float(np.linalg.norm(x64 - xref) / nref)}
+    # FP64-verified residual of the mixed solution
+    r_true32 = float(np.linalg.norm(Ap @ x32 - bp) / np.linalg.norm(bp))
+
+    # whole-iteration algorithmic bytes (solve-level GB/s)
+    b32 = sum(kernel_bytes(n, Ap.nnz, nnz_g, True))
+    b64 = sum(kernel_bytes(n, Ap.nnz, nnz_g, False))
+    solve32 = float(np.mean(d32))
+    solve64 = float(np.mean(d64))
+
+    dom = max(KERNEL_NAMES, key=lambda k: prof["mixed"][k]["avg_ms"])
+    dk = prof["mixed"][dom]
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}_mixed.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["gbs"], "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": dk["gbs"] / HBM_PEAK_GBS, "traffic": traffic,
+                "algorithmic_bytes": dk["bytes"], "avg_ms": dk["avg_ms"]}
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        O.set_threads(threads)
+        log(f"CPU baseline: oracle PCG, {args.cpu_iters} iterations on {threads} threads")
+        t = time.perf_counter()
+        O.pcg(Ap, bp, 1e-30, args.cpu_iters, Gs)
+        tc = time.perf_counter() - t
+        per_it = tc / args.cpu_iters
+        cpu = {"value": per_it * it64, "unit": "s", "cores": threads, "kind": "port",
+               "sample": (f"{args.cpu_iters} FP64 PCG iterations of the same {s}^3 system "
+                          f"(oracle/pcg_oracle.c: sequential triangular sweeps, OpenMP SpMV and "
+                          f"BLAS-1 on {threads} threads), {per_it:.2f} s/iteration scaled to the "
+                          f"{it64} iterations of the FP64 solve"),
+               "s_per_iter": per_it}
+        log(f"CPU: {per_it:.2f} s/iteration -> {per_it * it64:.1f} s")
+
+    res = {
+        "metric": "PCG time-to-1e-10 & HBM GB/s, 256^3 var-coef Poisson; mixed vs FP64 GPU",
+        "value": solve32,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": solve32 * 1e3,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64 (SpMV, dots, alpha/beta) + f32 (factor sweeps)",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{s}^3 variable-coefficient 7-point Poisson (face conductivity "
+                         f"exp(N(0,1.5^2)), +1e-3 I), ND {args.leaves} leaves, rchol factor "
+                         f"({args.order} order in blocks), b ~ U[-1,1], PCG to {args.tol:g}"),
+            "n": n, "nnz_a": int(Ap.nnz), "nnz_g": nnz_g, "fill": 2 * nnz_g / Ap.nnz,
+            "leaves": args.leaves, "order": args.order, "seeds": seeds,
+            "fwd_levels": st64["fwd_levels"], "bwd_levels": st64["bwd_levels"],
+            "factor_s": t_factor, "parallelism": "single GPU",
+        },
+        "mixed": {"time_s": solve32, "host_time_s": float(np.mean(t32)), "iters": it32,
+                  "relres": rr32, "relres_fp64_verified": r_true32,
+                  "hbm_gbs": b32 * it32 / solve32 / 1e9, "kernels": prof["mixed"]},
+        "fp64": {"time_s": solve64, "host_time_s": float(np.mean(t64)), "iters": it64,
+                 "relres": rr64, "hbm_gbs": b64 * it64 / solve64 / 1e9,
+                 "kernels": prof["fp64"]},
+        "speedup_mixed_vs_fp64": solve64 / solve32,
+        "error": err,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+    s32.close()
+    s64.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -42,6 +42,7 @@ typedef struct rchol_amd_ctx rchol_amd_ctx;
 #define RCHOL_AMD_OPT_RESID_REPLACE 2 /* true-residual replacement period (0 = never) */
 #define RCHOL_AMD_OPT_SPMV_LANES 3    /* lanes per row in SpMV (4, 8, 16 or 32) */
 #define RCHOL_AMD_OPT_TRSV_LANES 4    /* lanes per row in the triangular sweeps */
+#define RCHOL_AMD_OPT_PROFILE 5       /* 1: time every kernel of the loop with HIP events */
 
 /* Counters filled by rchol_amd_get_stats. */
 typedef struct rchol_amd_stats {
@@ -57,6 +58,11 @@ typedef struct rchol_amd_stats {
   int last_iters;
   double last_solve_ms;    /* device time of the last PCG loop (events) */
   double last_relres;
+  /* With RCHOL_AMD_OPT_PROFILE: summed device time of each loop kernel over
+   * the last solve -- [0] forward sweep, [1] backward sweep, [2] p update +
+   * SpMV, [3] x/r update -- and the iterations they cover. */
+  double kernel_ms[4];
+  int kernel_iters;
 } rchol_amd_stats;
 
 /* Thread-local text of the last error on the calling thread. */

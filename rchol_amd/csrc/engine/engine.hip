@@ -140,6 +140,9 @@ struct rchol_amd_ctx {
   int resid_period = 0;
   int spmv_lanes = 8;
   int trsv_lanes = 8;
+  int profile = 0;
+  int profile_no_graph = 0;
+  std::vector<hipEvent_t> pev;  // 5 per captured iteration when profiling
   // captured iteration graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -153,6 +156,7 @@ struct rchol_amd_ctx {
     if (hsc) (void)hipHostFree(hsc);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    for (hipEvent_t ev : pev) (void)hipEventDestroy(ev);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -174,7 +178,7 @@ struct rchol_amd_ctx {
   }
   int ensure_work();
   int size_trsv_grid();
-  int launch_iteration(int parity);
+  int launch_iteration(int parity, int slot);
   int pcg(const double *d_b, double tol, int maxit, double *d_x_out, int keep_x, double *relres,
           int *itr);
   int precond(const double *d_r, double *d_z);
@@ -186,7 +190,7 @@ namespace {
 // epilogues for the PCG loop; otherwise a standalone apply z = M^-1 r.
 template <typename VT, typename XT, int L>
 int launch_sweeps_t(rchol_amd_ctx *c, const double *rhs, XT *y, XT *z, const int *stop,
-                    bool with_dot, const double *dotv) {
+                    bool with_dot, const double *dotv, hipEvent_t *ev) {
   const VT *fcv, *fdg, *bcv, *bdg;
   if constexpr (sizeof(VT) == 8) {
     fcv = c->fwd.cv64.p; fdg = c->fwd.diag64.p; bcv = c->bwd.cv64.p; bdg = c->bwd.diag64.p;
@@ -196,6 +200,7 @@ int launch_sweeps_t(rchol_amd_ctx *c, const double *rhs, XT *y, XT *z, const int
   hipLaunchKernelGGL((k_sptrsv<VT, XT, double, L, 1>), dim3(c->trsv_grid), dim3(kBlock), 0,
                      c->stream, c->fwd.ntasks, c->fwd.task_ptr.p, c->fwd.slot_row.p, c->fwd.rp.p,
                      c->fwd.ci.p, fcv, fdg, rhs, y, z, nullptr, nullptr, c->sc, stop);
+  if (ev) (void)hipEventRecord(ev[1], c->stream);
   if (with_dot) {
     hipLaunchKernelGGL((k_sptrsv<VT, XT, XT, L, 2>), dim3(c->trsv_grid), dim3(kBlock), 0,
                        c->stream, c->bwd.ntasks, c->bwd.task_ptr.p, c->bwd.slot_row.p,
@@ -207,17 +212,18 @@ int launch_sweeps_t(rchol_amd_ctx *c, const double *rhs, XT *y, XT *z, const int
                        c->bwd.rp.p, c->bwd.ci.p, bcv, bdg, (const XT *)y, z, (XT *)nullptr,
                        nullptr, nullptr, c->sc, stop);
   }
+  if (ev) (void)hipEventRecord(ev[2], c->stream);
   return RCHOL_AMD_OK;
 }
 
 template <typename VT, typename XT>
 int dispatch_sweeps(rchol_amd_ctx *c, const double *rhs, XT *y, XT *z, const int *stop,
-                    bool with_dot, const double *dotv) {
+                    bool with_dot, const double *dotv, hipEvent_t *ev = nullptr) {
   switch (c->trsv_lanes) {
-    case 4: return launch_sweeps_t<VT, XT, 4>(c, rhs, y, z, stop, with_dot, dotv);
-    case 16: return launch_sweeps_t<VT, XT, 16>(c, rhs, y, z, stop, with_dot, dotv);
-    case 32: return launch_sweeps_t<VT, XT, 32>(c, rhs, y, z, stop, with_dot, dotv);
-    default: return launch_sweeps_t<VT, XT, 8>(c, rhs, y, z, stop, with_dot, dotv);
+    case 4: return launch_sweeps_t<VT, XT, 4>(c, rhs, y, z, stop, with_dot, dotv, ev);
+    case 16: return launch_sweeps_t<VT, XT, 16>(c, rhs, y, z, stop, with_dot, dotv, ev);
+    case 32: return launch_sweeps_t<VT, XT, 32>(c, rhs, y, z, stop, with_dot, dotv, ev);
+    default: return launch_sweeps_t<VT, XT, 8>(c, rhs, y, z, stop, with_dot, dotv, ev);
   }
 }
 
@@ -313,22 +319,27 @@ int rchol_amd_ctx::ensure_work() {
 
 // One PCG iteration (pcg.cpp:82-110 body): M^-1 r, p update + A p + alpha,
 // x/r update + loop test.  parity selects the p ping-pong buffers.
-int rchol_amd_ctx::launch_iteration(int parity) {
+int rchol_amd_ctx::launch_iteration(int parity, int slot) {
   double *p_old = parity ? p1.p : p0.p;
   double *p_new = parity ? p0.p : p1.p;
   const int *stop = &sc->done;
+  hipEvent_t *ev = profile ? &pev[(size_t)slot * 5] : nullptr;
+  if (ev) (void)hipEventRecord(ev[0], stream);
   if (gprec == RCHOL_AMD_FP32) {
-    dispatch_sweeps<float, float>(this, r.p, y32.p, z32.p, stop, true, r.p);
+    dispatch_sweeps<float, float>(this, r.p, y32.p, z32.p, stop, true, r.p, ev);
     launch_spmv_p<float>(this, z32.p, p_old, p_new);
+    if (ev) (void)hipEventRecord(ev[3], stream);
     hipLaunchKernelGGL((k_update<float>), dim3(vec_grid()), dim3(kBlock), 0, stream, (int)n,
                        (const double *)p_new, (const double *)q.p, x.p, r.p, y32.p, partials.p, sc);
   } else {
-    dispatch_sweeps<double, double>(this, r.p, y64.p, z64.p, stop, true, r.p);
+    dispatch_sweeps<double, double>(this, r.p, y64.p, z64.p, stop, true, r.p, ev);
     launch_spmv_p<double>(this, z64.p, p_old, p_new);
+    if (ev) (void)hipEventRecord(ev[3], stream);
     hipLaunchKernelGGL((k_update<double>), dim3(vec_grid()), dim3(kBlock), 0, stream, (int)n,
                        (const double *)p_new, (const double *)q.p, x.p, r.p, y64.p, partials.p,
                        sc);
   }
+  if (ev) (void)hipEventRecord(ev[4], stream);
   return RCHOL_AMD_OK;
 }
 
@@ -359,22 +370,49 @@ int rchol_amd_ctx::pcg(const double *d_b, double tol, int maxit, double *d_x_out
 
   int K = graph_iters > 0 ? graph_iters : 8;
   if (K & 1) K++;
-  if (graph_iters > 0 && !gexec) {
-    HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < K; i++) launch_iteration(i & 1);
-    hipError_t e = hipStreamEndCapture(stream, &graph);
-    if (e != hipSuccess) return fail(RCHOL_AMD_EHIP, "graph capture failed: %s", hipGetErrorString(e));
-    HIP_TRY(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+  if (profile && pev.size() < (size_t)K * 5) {
+    for (hipEvent_t ev : pev) (void)hipEventDestroy(ev);
+    pev.assign((size_t)K * 5, nullptr);
+    for (auto &ev : pev) HIP_TRY(hipEventCreate(&ev));
+    drop_graph();
   }
+  for (int i = 0; i < 4; i++) stats.kernel_ms[i] = 0.0;
+  stats.kernel_iters = 0;
+  if (graph_iters > 0 && !gexec && !(profile && profile_no_graph)) {
+    HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < K; i++) launch_iteration(i & 1, i);
+    hipError_t e = hipStreamEndCapture(stream, &graph);
+    if (e == hipSuccess) e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      drop_graph();
+      if (!profile) return fail(RCHOL_AMD_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+      profile_no_graph = 1;  // event nodes not capturable here: profile with direct launches
+    }
+  }
+  int k_seen = 0;
   for (;;) {
     if (gexec) {
       HIP_TRY(hipGraphLaunch(gexec, stream));
     } else {
-      for (int i = 0; i < K; i++) launch_iteration(i & 1);
+      for (int i = 0; i < K; i++) launch_iteration(i & 1, i);
       HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipMemcpyAsync(hsc, sc, sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
+    if (profile) {
+      // iterations that ran in this batch occupy slots 0 .. ran-1
+      const int ran = std::min(K, hsc->k - k_seen);
+      for (int i = 0; i < ran; i++) {
+        for (int kk = 0; kk < 4; kk++) {
+          float ms = 0.f;
+          HIP_TRY(hipEventElapsedTime(&ms, pev[(size_t)i * 5 + kk], pev[(size_t)i * 5 + kk + 1]));
+          stats.kernel_ms[kk] += ms;
+        }
+      }
+      stats.kernel_iters += ran;
+    }
+    k_seen = hsc->k;
     if (hsc->err) break;
     if (hsc->done) break;
   }
@@ -669,6 +707,9 @@ int rchol_amd_set_option(rchol_amd_ctx *c, int key, double value) {
     case RCHOL_AMD_OPT_TRSV_LANES:
       if (v != 4 && v != 8 && v != 16 && v != 32) return fail(RCHOL_AMD_EINVAL, "lanes must be 4/8/16/32");
       c->trsv_lanes = v;
+      break;
+    case RCHOL_AMD_OPT_PROFILE:
+      c->profile = v ? 1 : 0;
       break;
     default:
       return fail(RCHOL_AMD_EINVAL, "unknown option key %d", key);

@@ -18,6 +18,7 @@ OPT_GRAPH_ITERS = 1
 OPT_RESID_REPLACE = 2
 OPT_SPMV_LANES = 3
 OPT_TRSV_LANES = 4
+OPT_PROFILE = 5
 
 
 class Stats(ctypes.Structure):
@@ -34,10 +35,14 @@ class Stats(ctypes.Structure):
         ("last_iters", ctypes.c_int),
         ("last_solve_ms", ctypes.c_double),
         ("last_relres", ctypes.c_double),
+        ("kernel_ms", ctypes.c_double * 4),
+        ("kernel_iters", ctypes.c_int),
     ]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["kernel_ms"] = list(self.kernel_ms)
+        return d
 
 
 def _u64(a):
