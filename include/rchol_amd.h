@@ -111,6 +111,24 @@ int rchol_amd_pcg_device(rchol_amd_ctx *ctx, const double *d_b, double tol, int 
 /* Preconditioner apply on device pointers. */
 int rchol_amd_precond_device(rchol_amd_ctx *ctx, const double *d_r, double *d_z);
 
+/* Batched PCG on k right-hand sides (1 <= k <= 32), stored interleaved:
+ * B[i*k + c] is row i of column c (row-major n x k), X likewise.  Every
+ * column runs the recurrence of rchol_amd_pcg with its own scalars and stops
+ * on its own test; matrix entries are read once per row for all columns.
+ * relres[k], itr[k] receive the per-column results.  No reference
+ * counterpart: it generalises pcg::iteration (pcg.cpp:57-127) to a block
+ * of independent right-hand sides. */
+int rchol_amd_pcg_batch(rchol_amd_ctx *ctx, int k, const double *B, double tol, int maxit,
+                        double *X, int x_in_is_initial, double *relres, int *itr);
+
+/* Same on device pointers; k must be 4, 8, 16 or 32. */
+int rchol_amd_pcg_batch_device(rchol_amd_ctx *ctx, int k, const double *d_B, double tol,
+                               int maxit, double *d_X, int x_in_is_initial, double *relres,
+                               int *itr);
+
+/* Z = G^-T G^-1 R for k interleaved columns (pcg.cpp:141-159 per column). */
+int rchol_amd_precond_batch(rchol_amd_ctx *ctx, int k, const double *R, double *Z);
+
 /* One-shot solve with the exact argument list of the reference constructor
  * pcg(A, b, tol, maxit, G, x, relres, itr) (c++/util/pcg.hpp:15-16,
  * pcg.cpp:14-28): A as SparseCSR arrays, G as the stored factor arrays.

@@ -38,6 +38,12 @@ ENGINE_SYMBOLS = {
     "rchol_amd_pcg_csr": (ctypes.c_int, [ctypes.c_size_t, u64p, u64p, f64p, f64p,
                                          ctypes.c_double, ctypes.c_int, u64p, u64p, f64p, f64p,
                                          ctypes.c_int, f64p, i32p]),
+    "rchol_amd_pcg_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, f64p, ctypes.c_double,
+                                           ctypes.c_int, f64p, ctypes.c_int, f64p, i32p]),
+    "rchol_amd_pcg_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                  ctypes.c_double, ctypes.c_int, ctypes.c_void_p,
+                                                  ctypes.c_int, f64p, i32p]),
+    "rchol_amd_precond_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, f64p, f64p]),
     "rchol_amd_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]),
     "rchol_amd_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
 }

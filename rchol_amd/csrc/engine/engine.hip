@@ -19,6 +19,7 @@
 
 #include "../../../include/rchol_amd.h"
 #include "kernels.hpp"
+#include "kernels_mr.hpp"
 
 using namespace rchol_amd;
 
@@ -148,8 +149,29 @@ struct rchol_amd_ctx {
   hipGraphExec_t gexec = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   rchol_amd_stats stats{};
+  // multi-RHS work space (K interleaved columns)
+  struct Batch {
+    int K = 0;
+    size_t n = 0;
+    DevBuf<double> b, x, r, q, p0, p1, res, partials;
+    DevBuf<double> y64, z64;
+    DevBuf<float> y32, z32;
+    BatchScalars *bs = nullptr, *hbs = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int grid = 0;
+    void drop_graph() {
+      if (gexec) (void)hipGraphExecDestroy(gexec);
+      if (graph) (void)hipGraphDestroy(graph);
+      gexec = nullptr;
+      graph = nullptr;
+    }
+  } batch;
 
   ~rchol_amd_ctx() {
+    batch.drop_graph();
+    if (batch.bs) (void)hipFree(batch.bs);
+    if (batch.hbs) (void)hipHostFree(batch.hbs);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     if (sc) (void)hipFree(sc);
@@ -165,6 +187,7 @@ struct rchol_amd_ctx {
     if (graph) (void)hipGraphDestroy(graph);
     gexec = nullptr;
     graph = nullptr;
+    batch.drop_graph();
   }
 
   int vec_grid() const {
@@ -182,6 +205,17 @@ struct rchol_amd_ctx {
   int pcg(const double *d_b, double tol, int maxit, double *d_x_out, int keep_x, double *relres,
           int *itr);
   int precond(const double *d_r, double *d_z);
+  // multi-RHS
+  int ensure_batch(int K);
+  template <int K>
+  int pcg_batch(const double *d_b, int ncols, double tol, int maxit, double *d_x, int keep_x,
+                double *relres, int *itr);
+  template <int K>
+  int precond_batch(const double *d_r, double *d_z);
+  template <int K>
+  void launch_iteration_mr(int parity, int slot);
+  template <int K>
+  void launch_sweeps_mr(const double *rhs, bool with_dot, const int *stop, hipEvent_t *ev);
 };
 
 namespace {
@@ -463,6 +497,297 @@ int rchol_amd_ctx::precond(const double *d_r, double *d_z) {
   if (hsc->err)
     return fail(RCHOL_AMD_ESTALL, "triangular sweep wait exceeded its bound (err=%d)", hsc->err);
   return RCHOL_AMD_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Multi-RHS solve (K interleaved columns, per-column recurrences)
+// ---------------------------------------------------------------------------
+template <typename VT, typename XT, int K>
+static int trsv_occupancy_mr() {
+  int a = 0, b = 0, c = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &a, (const void *)k_sptrsv_mr<VT, XT, double, K, 1>, kBlock, 0) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &b, (const void *)k_sptrsv_mr<VT, XT, XT, K, 2>, kBlock, 0) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &c, (const void *)k_sptrsv_mr<VT, XT, XT, K, 0>, kBlock, 0) != hipSuccess)
+    return 0;
+  return std::min(a, std::min(b, c));
+}
+
+int rchol_amd_ctx::ensure_batch(int K) {
+  if (batch.K == K && batch.n == n && batch.b.p) return RCHOL_AMD_OK;
+  batch.drop_graph();
+  const size_t m = n * (size_t)K;
+  HIP_TRY(batch.b.alloc(m));
+  HIP_TRY(batch.x.alloc(m));
+  HIP_TRY(batch.r.alloc(m));
+  HIP_TRY(batch.q.alloc(m));
+  HIP_TRY(batch.p0.alloc(m));
+  HIP_TRY(batch.p1.alloc(m));
+  HIP_TRY(batch.res.alloc(m));
+  if (gprec == RCHOL_AMD_FP32) {
+    batch.y64.reset();
+    batch.z64.reset();
+    HIP_TRY(batch.y32.alloc(m));
+    HIP_TRY(batch.z32.alloc(m));
+  } else {
+    batch.y32.reset();
+    batch.z32.reset();
+    HIP_TRY(batch.y64.alloc(m));
+    HIP_TRY(batch.z64.alloc(m));
+  }
+  HIP_TRY(batch.partials.alloc((size_t)ncu * 8 * kMaxCols * 2 + 64));
+  if (!batch.bs) {
+    HIP_TRY(hipMalloc((void **)&batch.bs, sizeof(BatchScalars)));
+    HIP_TRY(hipMemset(batch.bs, 0, sizeof(BatchScalars)));
+    HIP_TRY(hipHostMalloc((void **)&batch.hbs, sizeof(BatchScalars), hipHostMallocDefault));
+    memset(batch.hbs, 0, sizeof(BatchScalars));
+  }
+  int per_cu = 0;
+  switch (K) {
+    case 4: per_cu = gprec == RCHOL_AMD_FP32 ? trsv_occupancy_mr<float, float, 4>() : trsv_occupancy_mr<double, double, 4>(); break;
+    case 8: per_cu = gprec == RCHOL_AMD_FP32 ? trsv_occupancy_mr<float, float, 8>() : trsv_occupancy_mr<double, double, 8>(); break;
+    case 16: per_cu = gprec == RCHOL_AMD_FP32 ? trsv_occupancy_mr<float, float, 16>() : trsv_occupancy_mr<double, double, 16>(); break;
+    default: per_cu = gprec == RCHOL_AMD_FP32 ? trsv_occupancy_mr<float, float, 32>() : trsv_occupancy_mr<double, double, 32>(); break;
+  }
+  if (per_cu <= 0) return fail(RCHOL_AMD_EHIP, "occupancy query for the batched sweeps failed");
+  per_cu = std::min(per_cu, 8);
+  const int max_tasks = std::max(fwd.ntasks, bwd.ntasks);
+  batch.grid = std::max(1, std::min(per_cu * ncu, (max_tasks + 3) / 4));
+  batch.K = K;
+  batch.n = n;
+  return RCHOL_AMD_OK;
+}
+
+template <int K>
+void rchol_amd_ctx::launch_sweeps_mr(const double *rhs, bool with_dot, const int *stop,
+                                     hipEvent_t *ev) {
+  const dim3 g(batch.grid), blk(kBlock);
+  if (gprec == RCHOL_AMD_FP32) {
+    hipLaunchKernelGGL((k_sptrsv_mr<float, float, double, K, 1>), g, blk, 0, stream, fwd.ntasks,
+                       fwd.task_ptr.p, fwd.slot_row.p, fwd.rp.p, fwd.ci.p, fwd.cv32.p,
+                       fwd.diag32.p, rhs, batch.y32.p, batch.z32.p, nullptr, nullptr, batch.bs,
+                       stop);
+    if (ev) (void)hipEventRecord(ev[1], stream);
+    if (with_dot)
+      hipLaunchKernelGGL((k_sptrsv_mr<float, float, float, K, 2>), g, blk, 0, stream, bwd.ntasks,
+                         bwd.task_ptr.p, bwd.slot_row.p, bwd.rp.p, bwd.ci.p, bwd.cv32.p,
+                         bwd.diag32.p, (const float *)batch.y32.p, batch.z32.p, (float *)nullptr,
+                         (const double *)batch.r.p, batch.partials.p, batch.bs, stop);
+    else
+      hipLaunchKernelGGL((k_sptrsv_mr<float, float, float, K, 0>), g, blk, 0, stream, bwd.ntasks,
+                         bwd.task_ptr.p, bwd.slot_row.p, bwd.rp.p, bwd.ci.p, bwd.cv32.p,
+                         bwd.diag32.p, (const float *)batch.y32.p, batch.z32.p, (float *)nullptr,
+                         nullptr, nullptr, batch.bs, stop);
+  } else {
+    hipLaunchKernelGGL((k_sptrsv_mr<double, double, double, K, 1>), g, blk, 0, stream,
+                       fwd.ntasks, fwd.task_ptr.p, fwd.slot_row.p, fwd.rp.p, fwd.ci.p,
+                       fwd.cv64.p, fwd.diag64.p, rhs, batch.y64.p, batch.z64.p, nullptr, nullptr,
+                       batch.bs, stop);
+    if (ev) (void)hipEventRecord(ev[1], stream);
+    if (with_dot)
+      hipLaunchKernelGGL((k_sptrsv_mr<double, double, double, K, 2>), g, blk, 0, stream,
+                         bwd.ntasks, bwd.task_ptr.p, bwd.slot_row.p, bwd.rp.p, bwd.ci.p,
+                         bwd.cv64.p, bwd.diag64.p, (const double *)batch.y64.p, batch.z64.p,
+                         (double *)nullptr, (const double *)batch.r.p, batch.partials.p,
+                         batch.bs, stop);
+    else
+      hipLaunchKernelGGL((k_sptrsv_mr<double, double, double, K, 0>), g, blk, 0, stream,
+                         bwd.ntasks, bwd.task_ptr.p, bwd.slot_row.p, bwd.rp.p, bwd.ci.p,
+                         bwd.cv64.p, bwd.diag64.p, (const double *)batch.y64.p, batch.z64.p,
+                         (double *)nullptr, nullptr, nullptr, batch.bs, stop);
+  }
+  if (ev) (void)hipEventRecord(ev[2], stream);
+}
+
+template <int K>
+void rchol_amd_ctx::launch_iteration_mr(int parity, int slot) {
+  double *p_old = parity ? batch.p1.p : batch.p0.p;
+  double *p_new = parity ? batch.p0.p : batch.p1.p;
+  hipEvent_t *ev = profile ? &pev[(size_t)slot * 5] : nullptr;
+  if (ev) (void)hipEventRecord(ev[0], stream);
+  launch_sweeps_mr<K>(batch.r.p, true, &batch.bs->all_done, ev);
+  const int rows_per_block = (kBlock / 64) * (64 / K);
+  const int sg = (int)std::max<size_t>(
+      1, std::min<size_t>((n + rows_per_block - 1) / rows_per_block, (size_t)ncu * 8));
+  const int vg = (int)std::max<size_t>(
+      1, std::min<size_t>((n * K + kBlock - 1) / kBlock, (size_t)ncu * 8));
+  if (gprec == RCHOL_AMD_FP32) {
+    hipLaunchKernelGGL((k_spmv_p_mr<float, K>), dim3(sg), dim3(kBlock), 0, stream, (int)n,
+                       arp.p, aci.p, av.p, (const float *)batch.z32.p, (const double *)p_old,
+                       p_new, batch.q.p, (const double *)batch.r.p, batch.partials.p, batch.bs);
+    if (ev) (void)hipEventRecord(ev[3], stream);
+    hipLaunchKernelGGL((k_update_mr<float, K>), dim3(vg), dim3(kBlock), 0, stream, (int)n,
+                       (const double *)p_new, (const double *)batch.q.p, batch.x.p, batch.r.p,
+                       batch.y32.p, batch.partials.p, batch.bs);
+  } else {
+    hipLaunchKernelGGL((k_spmv_p_mr<double, K>), dim3(sg), dim3(kBlock), 0, stream, (int)n,
+                       arp.p, aci.p, av.p, (const double *)batch.z64.p, (const double *)p_old,
+                       p_new, batch.q.p, (const double *)batch.r.p, batch.partials.p, batch.bs);
+    if (ev) (void)hipEventRecord(ev[3], stream);
+    hipLaunchKernelGGL((k_update_mr<double, K>), dim3(vg), dim3(kBlock), 0, stream, (int)n,
+                       (const double *)p_new, (const double *)batch.q.p, batch.x.p, batch.r.p,
+                       batch.y64.p, batch.partials.p, batch.bs);
+  }
+  if (ev) (void)hipEventRecord(ev[4], stream);
+}
+
+template <int K>
+int rchol_amd_ctx::pcg_batch(const double *d_b, int ncols, double tol, int maxit, double *d_x,
+                             int keep_x, double *relres, int *itr) {
+  if (!hasA || !hasG) return fail(RCHOL_AMD_ESTATE, "A and G must be uploaded before pcg");
+  if (nG != n) return fail(RCHOL_AMD_EINVAL, "G is %zu x %zu but A is %zu x %zu", nG, nG, n, n);
+  int rc = ensure_batch(K);
+  if (rc) return rc;
+  const size_t m = n * (size_t)K;
+  if (keep_x && d_x != batch.x.p)
+    HIP_TRY(hipMemcpyAsync(batch.x.p, d_x, m * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  batch.hbs->tol = tol;
+  batch.hbs->maxit = maxit;
+  batch.hbs->ncols = ncols;
+  batch.hbs->zero = 0;
+  HIP_TRY(hipMemcpyAsync(&batch.bs->tol, &batch.hbs->tol, sizeof(double), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(&batch.bs->maxit, &batch.hbs->maxit, sizeof(int), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(&batch.bs->ncols, &batch.hbs->ncols, sizeof(int), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(&batch.bs->zero, &batch.hbs->zero, sizeof(int), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipEventRecord(ev0, stream));
+  const int vg = (int)std::max<size_t>(1, std::min<size_t>((m + kBlock - 1) / kBlock, (size_t)ncu * 8));
+  if (gprec == RCHOL_AMD_FP32)
+    hipLaunchKernelGGL((k_init_mr<float, K>), dim3(vg), dim3(kBlock), 0, stream, (int)n, d_b,
+                       batch.r.p, batch.x.p, keep_x, batch.y32.p, batch.partials.p, batch.bs);
+  else
+    hipLaunchKernelGGL((k_init_mr<double, K>), dim3(vg), dim3(kBlock), 0, stream, (int)n, d_b,
+                       batch.r.p, batch.x.p, keep_x, batch.y64.p, batch.partials.p, batch.bs);
+  HIP_TRY(hipGetLastError());
+  int G = graph_iters > 0 ? graph_iters : 8;
+  if (G & 1) G++;
+  if (profile && pev.size() < (size_t)G * 5) {
+    for (hipEvent_t ev : pev) (void)hipEventDestroy(ev);
+    pev.assign((size_t)G * 5, nullptr);
+    for (auto &ev : pev) HIP_TRY(hipEventCreate(&ev));
+    drop_graph();
+  }
+  for (int i = 0; i < 4; i++) stats.kernel_ms[i] = 0.0;
+  stats.kernel_iters = 0;
+  if (graph_iters > 0 && !batch.gexec && !(profile && profile_no_graph)) {
+    HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < G; i++) launch_iteration_mr<K>(i & 1, i);
+    hipError_t e = hipStreamEndCapture(stream, &batch.graph);
+    if (e == hipSuccess) e = hipGraphInstantiate(&batch.gexec, batch.graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      batch.drop_graph();
+      if (!profile) return fail(RCHOL_AMD_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+      profile_no_graph = 1;
+    }
+  }
+  int maxk_seen = 0;
+  for (;;) {
+    if (batch.gexec) {
+      HIP_TRY(hipGraphLaunch(batch.gexec, stream));
+    } else {
+      for (int i = 0; i < G; i++) launch_iteration_mr<K>(i & 1, i);
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemcpyAsync(batch.hbs, batch.bs, sizeof(BatchScalars), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    int maxk = 0;
+    for (int c = 0; c < ncols; c++) maxk = std::max(maxk, batch.hbs->k[c]);
+    if (profile) {
+      const int ran = std::min(G, maxk - maxk_seen);
+      for (int i = 0; i < ran; i++)
+        for (int kk = 0; kk < 4; kk++) {
+          float ms = 0.f;
+          HIP_TRY(hipEventElapsedTime(&ms, pev[(size_t)i * 5 + kk], pev[(size_t)i * 5 + kk + 1]));
+          stats.kernel_ms[kk] += ms;
+        }
+      stats.kernel_iters += ran;
+    }
+    maxk_seen = maxk;
+    if (batch.hbs->err || batch.hbs->all_done) break;
+  }
+  if (batch.hbs->err)
+    return fail(RCHOL_AMD_ESTALL, "triangular sweep wait exceeded its bound (err=%d)", batch.hbs->err);
+  const int rows_per_block = (kBlock / 64) * (64 / K);
+  const int sg = (int)std::max<size_t>(
+      1, std::min<size_t>((n + rows_per_block - 1) / rows_per_block, (size_t)ncu * 8));
+  hipLaunchKernelGGL((k_spmv_res_mr<K>), dim3(sg), dim3(kBlock), 0, stream, (int)n, arp.p, aci.p,
+                     av.p, (const double *)batch.x.p, d_b, batch.res.p, batch.partials.p, batch.bs);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ev1, stream));
+  HIP_TRY(hipMemcpyAsync(batch.hbs, batch.bs, sizeof(BatchScalars), hipMemcpyDeviceToHost, stream));
+  if (d_x != batch.x.p)
+    HIP_TRY(hipMemcpyAsync(d_x, batch.x.p, m * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+  int maxk = 0;
+  for (int c = 0; c < ncols; c++) {
+    relres[c] = batch.hbs->relres[c];
+    itr[c] = batch.hbs->k[c];
+    maxk = std::max(maxk, itr[c]);
+  }
+  stats.last_iters = maxk;
+  stats.last_relres = ncols ? relres[0] : 0.0;
+  stats.last_solve_ms = ms;
+  return RCHOL_AMD_OK;
+}
+
+template <int K>
+int rchol_amd_ctx::precond_batch(const double *d_r, double *d_z) {
+  if (!hasG) return fail(RCHOL_AMD_ESTATE, "G must be uploaded before precond");
+  if (!hasA) n = nG;
+  if (n != nG) return fail(RCHOL_AMD_EINVAL, "G is %zu x %zu but A is %zu x %zu", nG, nG, n, n);
+  int rc = ensure_batch(K);
+  if (rc) return rc;
+  const size_t m = n * (size_t)K;
+  batch.hbs->zero = 0;
+  batch.hbs->err = 0;
+  HIP_TRY(hipMemcpyAsync(&batch.bs->zero, &batch.hbs->zero, sizeof(int), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(&batch.bs->err, &batch.hbs->err, sizeof(int), hipMemcpyHostToDevice, stream));
+  const int g = (int)std::max<size_t>(1, std::min<size_t>((m + kBlock - 1) / kBlock, (size_t)ncu * 8));
+  if (gprec == RCHOL_AMD_FP32) {
+    hipLaunchKernelGGL((k_fill_sentinel<float>), dim3(g), dim3(kBlock), 0, stream, (int)m, batch.y32.p);
+    launch_sweeps_mr<K>(d_r, false, &batch.bs->zero, nullptr);
+    hipLaunchKernelGGL(k_widen, dim3(g), dim3(kBlock), 0, stream, (int)m, (const float *)batch.z32.p, d_z);
+  } else {
+    hipLaunchKernelGGL((k_fill_sentinel<double>), dim3(g), dim3(kBlock), 0, stream, (int)m, batch.y64.p);
+    launch_sweeps_mr<K>(d_r, false, &batch.bs->zero, nullptr);
+    HIP_TRY(hipMemcpyAsync(d_z, batch.z64.p, m * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(&batch.hbs->err, &batch.bs->err, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  if (batch.hbs->err)
+    return fail(RCHOL_AMD_ESTALL, "triangular sweep wait exceeded its bound (err=%d)", batch.hbs->err);
+  return RCHOL_AMD_OK;
+}
+
+static int batch_width(int k) {
+  if (k <= 4) return 4;
+  if (k <= 8) return 8;
+  if (k <= 16) return 16;
+  return 32;
+}
+
+// Dispatch on the compile-time width.
+#define RCHOL_BATCH_DISPATCH(K, CALL)                  \
+  switch (K) {                                          \
+    case 4: { constexpr int KK = 4; return c->CALL; }   \
+    case 8: { constexpr int KK = 8; return c->CALL; }   \
+    case 16: { constexpr int KK = 16; return c->CALL; } \
+    default: { constexpr int KK = 32; return c->CALL; } \
+  }
+
+static int pcg_batch_dev(rchol_amd_ctx *c, int K, const double *d_b, int k, double tol, int maxit,
+                         double *d_x, int keep, double *relres, int *itr) {
+  RCHOL_BATCH_DISPATCH(K, template pcg_batch<KK>(d_b, k, tol, maxit, d_x, keep, relres, itr));
+}
+
+static int precond_batch_dev(rchol_amd_ctx *c, int K, const double *d_r, double *d_z) {
+  RCHOL_BATCH_DISPATCH(K, template precond_batch<KK>(d_r, d_z));
 }
 
 // ---------------------------------------------------------------------------
@@ -781,6 +1106,74 @@ int rchol_amd_pcg(rchol_amd_ctx *c, const double *bh, double tol, int maxit, dou
   rc = c->pcg(c->b.p, tol, maxit, c->x.p, x_in_is_initial, relres, itr);
   if (rc) return rc;
   HIP_TRY(hipMemcpy(xh, c->x.p, c->n * sizeof(double), hipMemcpyDeviceToHost));
+  return RCHOL_AMD_OK;
+}
+
+
+int rchol_amd_pcg_batch(rchol_amd_ctx *c, int k, const double *B, double tol, int maxit,
+                        double *X, int x_in_is_initial, double *relres, int *itr) {
+  if (!c || !B || !X || !relres || !itr) return fail(RCHOL_AMD_EINVAL, "null argument");
+  if (k < 1 || k > kMaxCols) return fail(RCHOL_AMD_EINVAL, "batch width %d outside [1, %d]", k, kMaxCols);
+  if (!c->hasA) return fail(RCHOL_AMD_ESTATE, "A must be uploaded before pcg");
+  HIP_TRY(hipSetDevice(c->device));
+  const int K = batch_width(k);
+  const size_t n = c->n, m = n * (size_t)K;
+  int rc = c->ensure_batch(K);
+  if (rc) return rc;
+  // pack the caller's n x k interleaved block into n x K
+  std::vector<double> hb(m, 0.0), hx;
+  for (size_t i = 0; i < n; i++)
+    for (int j = 0; j < k; j++) hb[i * K + j] = B[i * k + j];
+  HIP_TRY(hipMemcpyAsync(c->batch.b.p, hb.data(), m * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (x_in_is_initial) {
+    hx.assign(m, 0.0);
+    for (size_t i = 0; i < n; i++)
+      for (int j = 0; j < k; j++) hx[i * K + j] = X[i * k + j];
+    HIP_TRY(hipMemcpyAsync(c->batch.x.p, hx.data(), m * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  }
+  std::vector<double> rr(kMaxCols);
+  std::vector<int> it(kMaxCols);
+  rc = pcg_batch_dev(c, K, c->batch.b.p, k, tol, maxit, c->batch.x.p, x_in_is_initial, rr.data(), it.data());
+  if (rc) return rc;
+  hx.assign(m, 0.0);
+  HIP_TRY(hipMemcpy(hx.data(), c->batch.x.p, m * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; i++)
+    for (int j = 0; j < k; j++) X[i * k + j] = hx[i * K + j];
+  for (int j = 0; j < k; j++) {
+    relres[j] = rr[j];
+    itr[j] = it[j];
+  }
+  return RCHOL_AMD_OK;
+}
+
+int rchol_amd_pcg_batch_device(rchol_amd_ctx *c, int k, const double *d_B, double tol, int maxit,
+                               double *d_X, int x_in_is_initial, double *relres, int *itr) {
+  if (!c || !d_B || !d_X || !relres || !itr) return fail(RCHOL_AMD_EINVAL, "null argument");
+  if (k != 4 && k != 8 && k != 16 && k != 32)
+    return fail(RCHOL_AMD_EINVAL, "device batches take 4, 8, 16 or 32 columns (got %d)", k);
+  HIP_TRY(hipSetDevice(c->device));
+  return pcg_batch_dev(c, k, d_B, k, tol, maxit, d_X, x_in_is_initial, relres, itr);
+}
+
+int rchol_amd_precond_batch(rchol_amd_ctx *c, int k, const double *R, double *Z) {
+  if (!c || !R || !Z) return fail(RCHOL_AMD_EINVAL, "null argument");
+  if (k < 1 || k > kMaxCols) return fail(RCHOL_AMD_EINVAL, "batch width %d outside [1, %d]", k, kMaxCols);
+  if (!c->hasG) return fail(RCHOL_AMD_ESTATE, "G must be uploaded before precond");
+  HIP_TRY(hipSetDevice(c->device));
+  if (!c->hasA) c->n = c->nG;
+  const int K = batch_width(k);
+  const size_t n = c->nG, m = n * (size_t)K;
+  int rc = c->ensure_batch(K);
+  if (rc) return rc;
+  std::vector<double> h(m, 0.0);
+  for (size_t i = 0; i < n; i++)
+    for (int j = 0; j < k; j++) h[i * K + j] = R[i * k + j];
+  HIP_TRY(hipMemcpy(c->batch.b.p, h.data(), m * sizeof(double), hipMemcpyHostToDevice));
+  rc = precond_batch_dev(c, K, c->batch.b.p, c->batch.res.p);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(h.data(), c->batch.res.p, m * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; i++)
+    for (int j = 0; j < k; j++) Z[i * k + j] = h[i * K + j];
   return RCHOL_AMD_OK;
 }
 

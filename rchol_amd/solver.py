@@ -165,6 +165,44 @@ class Solver:
                                            ctypes.byref(relres), ctypes.byref(itr)))
         return x, relres.value, itr.value
 
+    def pcg_batch(self, B, tol, maxit, X0=None):
+        """k independent right-hand sides (columns of B, n x k, k <= 32) solved
+        together -> (X, relres[k], itr[k])."""
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        n, k = B.shape
+        keep = 0
+        X = np.zeros_like(B)
+        if X0 is not None:
+            X[:] = X0
+            keep = 1
+        rr = np.zeros(k)
+        it = np.zeros(k, dtype=np.int32)
+        _lib.check(self._lib.rchol_amd_pcg_batch(self._h, int(k), B.ctypes.data_as(_lib.f64p),
+                                                 float(tol), int(maxit),
+                                                 X.ctypes.data_as(_lib.f64p), keep,
+                                                 rr.ctypes.data_as(_lib.f64p),
+                                                 it.ctypes.data_as(_lib.i32p)))
+        return X, rr, it
+
+    def precond_batch(self, R):
+        R = np.ascontiguousarray(R, dtype=np.float64)
+        n, k = R.shape
+        Z = np.empty_like(R)
+        _lib.check(self._lib.rchol_amd_precond_batch(self._h, int(k), R.ctypes.data_as(_lib.f64p),
+                                                     Z.ctypes.data_as(_lib.f64p)))
+        return Z
+
+    def pcg_batch_device(self, k, b_ptr, x_ptr, tol, maxit, x_is_initial=False):
+        rr = np.zeros(k)
+        it = np.zeros(k, dtype=np.int32)
+        _lib.check(self._lib.rchol_amd_pcg_batch_device(self._h, int(k), ctypes.c_void_p(b_ptr),
+                                                        float(tol), int(maxit),
+                                                        ctypes.c_void_p(x_ptr),
+                                                        int(bool(x_is_initial)),
+                                                        rr.ctypes.data_as(_lib.f64p),
+                                                        it.ctypes.data_as(_lib.i32p)))
+        return rr, it
+
     def pcg_device(self, b_ptr, x_ptr, tol, maxit, x_is_initial=False):
         """PCG on device pointers (e.g. torch tensors' data_ptr())."""
         relres = ctypes.c_double()

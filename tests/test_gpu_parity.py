@@ -158,3 +158,48 @@ def test_upload_rejects_bad_factor():
         bad = (np.array([0, 1, 2]), np.array([1, 1]), np.array([1.0, 1.0]))
         with pytest.raises(RuntimeError):
             s.upload_G(bad)  # column 0 misses its diagonal
+
+
+@pytest.mark.parametrize("name", ["lap3d_8_nd4", "varcoef_6_nd2", "lap3d_12_nd8"])
+@pytest.mark.parametrize("k", [3, 8, 32])
+@pytest.mark.parametrize("precision", [rchol_amd.FP64, rchol_amd.FP32])
+def test_batch_pcg_matches_single_column_oracle(name, k, precision):
+    A, Gs, d = load(name)
+    n = A.shape[0]
+    print(f"case batch {name} k={k} prec={precision}", flush=True)
+    rng = np.random.default_rng(k)
+    B = rng.uniform(-1, 1, (n, k))
+    B[:, 0] = d["b"]
+    with rchol_amd.Solver(A, Gs, precision=precision) as s:
+        X, rr, it = s.pcg_batch(B, 1e-10, 5000)
+        Z = s.precond_batch(B)
+    for c in range(k):
+        xo, ro, io = O.pcg(A, B[:, c], 1e-10, 5000, Gs)
+        zo = O.precond(Gs, B[:, c])
+        true = np.linalg.norm(A @ X[:, c] - B[:, c]) / np.linalg.norm(B[:, c])
+        if precision == rchol_amd.FP64:
+            assert abs(int(it[c]) - io) <= 1
+            assert rel(Z[:, c], zo) <= PRECOND_RTOL
+            assert rel(X[:, c], xo) <= 1e-8
+        else:
+            assert rel(Z[:, c], zo) <= 1e-5
+            assert int(it[c]) <= io + 5
+        assert true <= 1e-9
+        assert rr[c] == pytest.approx(true, rel=1e-6)
+
+
+def test_batch_columns_stop_independently():
+    A, Gs, d = load("lap3d_8_nd4")
+    n = A.shape[0]
+    print("case batch_masks", flush=True)
+    B = np.zeros((n, 4))
+    B[:, 0] = d["b"]
+    B[:, 1] = 1e-3 * d["b"]  # same direction, same iteration count
+    B[:, 3] = np.random.default_rng(0).uniform(-1, 1, n)
+    # column 2 is zero: no iteration, x stays 0
+    with rchol_amd.Solver(A, Gs) as s:
+        X, rr, it = s.pcg_batch(B, 1e-10, 5000)
+        x0, _, i0 = s.pcg(d["b"], 1e-10, 5000)
+    assert it[2] == 0 and np.all(X[:, 2] == 0)
+    assert it[0] == i0 and abs(int(it[1]) - i0) <= 1
+    assert rel(X[:, 0], x0) <= 1e-12
