@@ -138,6 +138,22 @@ int rchol_amd_pcg_csr(size_t n, const size_t *aRowPtr, const size_t *aColIdx,
                       const size_t *gColPtr, const size_t *gRowIdx, const double *gVal,
                       double *x, int x_in_is_initial, double *relres, int *itr);
 
+/* Declare the ordering P of the uploaded A and G (they factor A(P,P), as
+ * returned by rchol(A, G, P, threads), rchol_parallel.hpp:6): from then on
+ * the vectors passed to pcg / precond (single and batched) are in the
+ * original numbering; the engine gathers b(P) (util.hpp:147-155) and
+ * scatters x(P) = x' (ex_laplace.py:30-31) on the device.  perm == NULL
+ * clears it. */
+int rchol_amd_set_perm(rchol_amd_ctx *ctx, size_t n, const size_t *perm);
+
+/* g[e] = -(lam[u[e]] - lam[v[e]]) * (x[u[e]] - x[v[e]]) on device pointers:
+ * the gradient of a loss through x = A(w)^-1 b with respect to the weight of
+ * edge (u, v) of a weighted Laplacian A(w), given lam = A^-1 dL/dx.  No
+ * reference counterpart (used by the differentiable solve). */
+int rchol_amd_edge_grad_device(rchol_amd_ctx *ctx, size_t m, const long long *d_u,
+                               const long long *d_v, const double *d_lam, const double *d_x,
+                               double *d_g);
+
 int rchol_amd_set_option(rchol_amd_ctx *ctx, int key, double value);
 int rchol_amd_get_stats(rchol_amd_ctx *ctx, rchol_amd_stats *out);
 

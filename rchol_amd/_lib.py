@@ -44,6 +44,11 @@ ENGINE_SYMBOLS = {
                                                   ctypes.c_double, ctypes.c_int, ctypes.c_void_p,
                                                   ctypes.c_int, f64p, i32p]),
     "rchol_amd_precond_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, f64p, f64p]),
+    "rchol_amd_set_perm": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, u64p]),
+    "rchol_amd_edge_grad_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t,
+                                                  ctypes.c_void_p, ctypes.c_void_p,
+                                                  ctypes.c_void_p, ctypes.c_void_p,
+                                                  ctypes.c_void_p]),
     "rchol_amd_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]),
     "rchol_amd_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
 }

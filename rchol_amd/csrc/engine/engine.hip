@@ -129,6 +129,11 @@ struct rchol_amd_ctx {
   int gprec = RCHOL_AMD_FP64;
   Sweep fwd, bwd;
   int trsv_grid = 0;
+  // optional ordering: vectors handed to the solves are in the original
+  // numbering, A and G in the permuted one (pcg runs on A(P,P))
+  bool hasPerm = false;
+  DevBuf<int> perm;
+  DevBuf<double> pin, pout;  // gathered input / permuted output staging
   // work vectors
   size_t nwork = 0;
   DevBuf<double> b, x, r, q, p0, p1, partials, tmp64a;
@@ -1052,10 +1057,73 @@ int rchol_amd_get_stats(rchol_amd_ctx *c, rchol_amd_stats *out) {
   return RCHOL_AMD_OK;
 }
 
+namespace {
+// Gather (in -> staging, permuted numbering) or scatter (staging -> out).
+int permute_rows(rchol_amd_ctx *c, bool gather, const double *in, double *out, int K) {
+  const size_t total = c->nG * (size_t)K;
+  const int g = (int)std::max<size_t>(1, std::min<size_t>((total + kBlock - 1) / kBlock, (size_t)c->ncu * 8));
+  if (gather)
+    hipLaunchKernelGGL(k_permute<1>, dim3(g), dim3(kBlock), 0, c->stream, (int)c->nG, K,
+                       (const int *)c->perm.p, in, out);
+  else
+    hipLaunchKernelGGL(k_permute<0>, dim3(g), dim3(kBlock), 0, c->stream, (int)c->nG, K,
+                       (const int *)c->perm.p, in, out);
+  HIP_TRY(hipGetLastError());
+  return RCHOL_AMD_OK;
+}
+
+int ensure_staging(rchol_amd_ctx *c, size_t m) {
+  if (c->pin.n < m) HIP_TRY(c->pin.alloc(m));
+  if (c->pout.n < m) HIP_TRY(c->pout.alloc(m));
+  return RCHOL_AMD_OK;
+}
+}  // namespace
+
 int rchol_amd_precond_device(rchol_amd_ctx *c, const double *d_r, double *d_z) {
   if (!c || !d_r || !d_z) return fail(RCHOL_AMD_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(c->device));
-  return c->precond(d_r, d_z);
+  if (!c->hasPerm) return c->precond(d_r, d_z);
+  int rc = ensure_staging(c, c->nG);
+  if (!rc) rc = permute_rows(c, true, d_r, c->pin.p, 1);
+  if (!rc) rc = c->precond(c->pin.p, c->pout.p);
+  if (!rc) rc = permute_rows(c, false, c->pout.p, d_z, 1);
+  if (!rc) HIP_TRY(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+int rchol_amd_set_perm(rchol_amd_ctx *c, size_t n, const size_t *perm) {
+  if (!c) return fail(RCHOL_AMD_EINVAL, "null context");
+  if (!perm) {
+    c->hasPerm = false;
+    c->perm.reset();
+    return RCHOL_AMD_OK;
+  }
+  if (n == 0 || n > kIdxMax) return fail(RCHOL_AMD_ERANGE, "n=%zu outside [1, 2^31)", n);
+  std::vector<int> p(n);
+  std::vector<char> seen(n, 0);
+  for (size_t i = 0; i < n; i++) {
+    if (perm[i] >= n || seen[perm[i]]) return fail(RCHOL_AMD_EINVAL, "perm is not a permutation of 0..%zu", n - 1);
+    seen[perm[i]] = 1;
+    p[i] = (int)perm[i];
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(c->perm.upload(p.data(), n));
+  c->hasPerm = true;
+  return RCHOL_AMD_OK;
+}
+
+int rchol_amd_edge_grad_device(rchol_amd_ctx *c, size_t m, const long long *d_u,
+                               const long long *d_v, const double *d_lam, const double *d_x,
+                               double *d_g) {
+  if (!c || (m && (!d_u || !d_v || !d_lam || !d_x || !d_g))) return fail(RCHOL_AMD_EINVAL, "null argument");
+  if (m == 0) return RCHOL_AMD_OK;
+  HIP_TRY(hipSetDevice(c->device));
+  const int g = (int)std::max<size_t>(1, std::min<size_t>((m + kBlock - 1) / kBlock, (size_t)c->ncu * 8));
+  hipLaunchKernelGGL(k_edge_grad, dim3(g), dim3(kBlock), 0, c->stream, m, d_u, d_v, d_lam, d_x, d_g);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RCHOL_AMD_OK;
 }
 
 int rchol_amd_precond(rchol_amd_ctx *c, const double *r, double *z) {
@@ -1066,7 +1134,7 @@ int rchol_amd_precond(rchol_amd_ctx *c, const double *r, double *z) {
   DevBuf<double> dr, dz;
   HIP_TRY(dr.upload(r, n));
   HIP_TRY(dz.alloc(n));
-  int rc = c->precond(dr.p, dz.p);
+  int rc = rchol_amd_precond_device(c, dr.p, dz.p);
   if (rc) return rc;
   HIP_TRY(hipMemcpy(z, dz.p, n * sizeof(double), hipMemcpyDeviceToHost));
   return RCHOL_AMD_OK;
@@ -1090,7 +1158,15 @@ int rchol_amd_pcg_device(rchol_amd_ctx *c, const double *d_b, double tol, int ma
                          int x_in_is_initial, double *relres, int *itr) {
   if (!c || !d_b || !d_x || !relres || !itr) return fail(RCHOL_AMD_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(c->device));
-  return c->pcg(d_b, tol, maxit, d_x, x_in_is_initial, relres, itr);
+  if (!c->hasPerm) return c->pcg(d_b, tol, maxit, d_x, x_in_is_initial, relres, itr);
+  if (c->nG != c->n) return fail(RCHOL_AMD_EINVAL, "perm size does not match A");
+  int rc = ensure_staging(c, c->n);
+  if (!rc) rc = permute_rows(c, true, d_b, c->pin.p, 1);
+  if (!rc && x_in_is_initial) rc = permute_rows(c, true, d_x, c->pout.p, 1);
+  if (!rc) rc = c->pcg(c->pin.p, tol, maxit, c->pout.p, x_in_is_initial, relres, itr);
+  if (!rc) rc = permute_rows(c, false, c->pout.p, d_x, 1);
+  if (!rc) HIP_TRY(hipStreamSynchronize(c->stream));
+  return rc;
 }
 
 int rchol_amd_pcg(rchol_amd_ctx *c, const double *bh, double tol, int maxit, double *xh,
@@ -1100,6 +1176,18 @@ int rchol_amd_pcg(rchol_amd_ctx *c, const double *bh, double tol, int maxit, dou
   HIP_TRY(hipSetDevice(c->device));
   int rc = c->ensure_work();
   if (rc) return rc;
+  if (c->hasPerm) {
+    DevBuf<double> db, dx;
+    HIP_TRY(db.upload(bh, c->n));
+    if (x_in_is_initial)
+      HIP_TRY(dx.upload(xh, c->n));
+    else
+      HIP_TRY(dx.alloc(c->n));
+    rc = rchol_amd_pcg_device(c, db.p, tol, maxit, dx.p, x_in_is_initial, relres, itr);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(xh, dx.p, c->n * sizeof(double), hipMemcpyDeviceToHost));
+    return RCHOL_AMD_OK;
+  }
   HIP_TRY(hipMemcpyAsync(c->b.p, bh, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   if (x_in_is_initial)
     HIP_TRY(hipMemcpyAsync(c->x.p, xh, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -1122,13 +1210,21 @@ int rchol_amd_pcg_batch(rchol_amd_ctx *c, int k, const double *B, double tol, in
   if (rc) return rc;
   // pack the caller's n x k interleaved block into n x K
   std::vector<double> hb(m, 0.0), hx;
+  std::vector<size_t> P(n);
+  if (c->hasPerm) {
+    std::vector<int> hp(n);
+    HIP_TRY(hipMemcpy(hp.data(), c->perm.p, n * sizeof(int), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) P[i] = (size_t)hp[i];
+  } else {
+    for (size_t i = 0; i < n; i++) P[i] = i;
+  }
   for (size_t i = 0; i < n; i++)
-    for (int j = 0; j < k; j++) hb[i * K + j] = B[i * k + j];
+    for (int j = 0; j < k; j++) hb[i * K + j] = B[P[i] * k + j];
   HIP_TRY(hipMemcpyAsync(c->batch.b.p, hb.data(), m * sizeof(double), hipMemcpyHostToDevice, c->stream));
   if (x_in_is_initial) {
     hx.assign(m, 0.0);
     for (size_t i = 0; i < n; i++)
-      for (int j = 0; j < k; j++) hx[i * K + j] = X[i * k + j];
+      for (int j = 0; j < k; j++) hx[i * K + j] = X[P[i] * k + j];
     HIP_TRY(hipMemcpyAsync(c->batch.x.p, hx.data(), m * sizeof(double), hipMemcpyHostToDevice, c->stream));
   }
   std::vector<double> rr(kMaxCols);
@@ -1138,7 +1234,7 @@ int rchol_amd_pcg_batch(rchol_amd_ctx *c, int k, const double *B, double tol, in
   hx.assign(m, 0.0);
   HIP_TRY(hipMemcpy(hx.data(), c->batch.x.p, m * sizeof(double), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < n; i++)
-    for (int j = 0; j < k; j++) X[i * k + j] = hx[i * K + j];
+    for (int j = 0; j < k; j++) X[P[i] * k + j] = hx[i * K + j];
   for (int j = 0; j < k; j++) {
     relres[j] = rr[j];
     itr[j] = it[j];
@@ -1152,7 +1248,15 @@ int rchol_amd_pcg_batch_device(rchol_amd_ctx *c, int k, const double *d_B, doubl
   if (k != 4 && k != 8 && k != 16 && k != 32)
     return fail(RCHOL_AMD_EINVAL, "device batches take 4, 8, 16 or 32 columns (got %d)", k);
   HIP_TRY(hipSetDevice(c->device));
-  return pcg_batch_dev(c, k, d_B, k, tol, maxit, d_X, x_in_is_initial, relres, itr);
+  if (!c->hasPerm) return pcg_batch_dev(c, k, d_B, k, tol, maxit, d_X, x_in_is_initial, relres, itr);
+  const size_t m = c->n * (size_t)k;
+  int rc = ensure_staging(c, m);
+  if (!rc) rc = permute_rows(c, true, d_B, c->pin.p, k);
+  if (!rc && x_in_is_initial) rc = permute_rows(c, true, d_X, c->pout.p, k);
+  if (!rc) rc = pcg_batch_dev(c, k, c->pin.p, k, tol, maxit, c->pout.p, x_in_is_initial, relres, itr);
+  if (!rc) rc = permute_rows(c, false, c->pout.p, d_X, k);
+  if (!rc) HIP_TRY(hipStreamSynchronize(c->stream));
+  return rc;
 }
 
 int rchol_amd_precond_batch(rchol_amd_ctx *c, int k, const double *R, double *Z) {
@@ -1166,14 +1270,22 @@ int rchol_amd_precond_batch(rchol_amd_ctx *c, int k, const double *R, double *Z)
   int rc = c->ensure_batch(K);
   if (rc) return rc;
   std::vector<double> h(m, 0.0);
+  std::vector<size_t> P(n);
+  if (c->hasPerm) {
+    std::vector<int> hp(n);
+    HIP_TRY(hipMemcpy(hp.data(), c->perm.p, n * sizeof(int), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) P[i] = (size_t)hp[i];
+  } else {
+    for (size_t i = 0; i < n; i++) P[i] = i;
+  }
   for (size_t i = 0; i < n; i++)
-    for (int j = 0; j < k; j++) h[i * K + j] = R[i * k + j];
+    for (int j = 0; j < k; j++) h[i * K + j] = R[P[i] * k + j];
   HIP_TRY(hipMemcpy(c->batch.b.p, h.data(), m * sizeof(double), hipMemcpyHostToDevice));
   rc = precond_batch_dev(c, K, c->batch.b.p, c->batch.res.p);
   if (rc) return rc;
   HIP_TRY(hipMemcpy(h.data(), c->batch.res.p, m * sizeof(double), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < n; i++)
-    for (int j = 0; j < k; j++) Z[i * k + j] = h[i * K + j];
+    for (int j = 0; j < k; j++) Z[P[i] * k + j] = h[i * K + j];
   return RCHOL_AMD_OK;
 }
 

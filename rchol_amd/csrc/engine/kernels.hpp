@@ -88,6 +88,37 @@ __global__ void k_widen(int n, const float *__restrict__ a, double *__restrict__
     b[i] = (double)a[i];
 }
 
+// Row permutation of K-wide interleaved blocks: out[i] = in[perm[i]]
+// (gather, the reference's reorder(x, P, xp), util.hpp:147-155) or
+// out[perm[i]] = in[i] (scatter, y[p] = x of ex_laplace.py:30-31).
+template <int GATHER>
+__global__ void k_permute(int n, int K, const int *__restrict__ perm, const double *__restrict__ in,
+                          double *__restrict__ out) {
+  const size_t total = (size_t)n * K;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t i = t / K, c = t % K;
+    const size_t p = (size_t)perm[i] * K + c;
+    if (GATHER)
+      out[t] = in[p];
+    else
+      out[p] = in[t];
+  }
+}
+
+// Gradient of a loss through x = A(w)^-1 b with respect to the weight w_e of
+// edge (u_e, v_e) of A = sum_e w_e (e_u - e_v)(e_u - e_v)^T + D:
+// dL/dw_e = -(lam_u - lam_v)(x_u - x_v), lam = A^-1 dL/dx.
+__global__ void k_edge_grad(size_t m, const long long *__restrict__ u, const long long *__restrict__ v,
+                            const double *__restrict__ lam, const double *__restrict__ x,
+                            double *__restrict__ g) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < m;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const long long a = u[e], b = v[e];
+    g[e] = -(lam[a] - lam[b]) * (x[a] - x[b]);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // PCG vector stages.  Grid-stride, a fixed grid so that the last-block
 // reductions see a bounded number of partials.

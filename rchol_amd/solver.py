@@ -136,6 +136,20 @@ class Solver:
         if self.n is None:
             self.n = n
 
+    def set_perm(self, perm):
+        """Vectors of later solves are in the original numbering; A and G
+        factor A(perm, perm) (perm=None clears)."""
+        if perm is None:
+            _lib.check(self._lib.rchol_amd_set_perm(self._h, 0, None))
+            return
+        p, pp = _u64(perm)
+        _lib.check(self._lib.rchol_amd_set_perm(self._h, p.size, pp))
+
+    def edge_grad_device(self, m, u_ptr, v_ptr, lam_ptr, x_ptr, g_ptr):
+        _lib.check(self._lib.rchol_amd_edge_grad_device(
+            self._h, int(m), ctypes.c_void_p(u_ptr), ctypes.c_void_p(v_ptr),
+            ctypes.c_void_p(lam_ptr), ctypes.c_void_p(x_ptr), ctypes.c_void_p(g_ptr)))
+
     def precond(self, r):
         """z = G^-T G^-1 r (pcg.cpp:141-159)."""
         r, pr = _f64(r)
