@@ -18,20 +18,32 @@ def softplus(t):
     return np.log1p(np.exp(-np.abs(t))) + np.maximum(t, 0)
 
 
-def refined_solve(A, b, steps=4):
-    """FP64 LU solve refined with extended-precision (long double) residuals,
-    so that finite differences of the loss are not swamped by the solve's
-    rounding (cond(A) * eps)."""
+def softplus_ld(t):
+    t = np.asarray(t, dtype=np.longdouble)
+    return np.log1p(np.exp(-np.abs(t))) + np.maximum(t, 0)
+
+
+def laplacian_matvec_ld(u, v, w, shift, x):
+    """(L(w) + shift I) x in long double, from the edge list."""
+    d = x[u] - x[v]
+    f = w * d
+    y = shift * x
+    np.add.at(y, u, f)
+    np.add.at(y, v, -f)
+    return y
+
+
+def refined_solve(A, matvec_ld, b, steps=6):
+    """Solve with an FP64 LU of A refined against the long-double operator
+    matvec_ld (weights and residuals in extended precision), so that central
+    differences of the loss are not swamped by FP64 assembly and solve
+    rounding (~cond(A) * eps)."""
     import scipy.sparse as sp
-    As = sp.csr_matrix(A)
-    lu = spl.splu(sp.csc_matrix(As))
-    data = As.data.astype(np.longdouble)
-    idx, ptr = As.indices, As.indptr
+    lu = spl.splu(sp.csc_matrix(A))
     bl = np.asarray(b, dtype=np.longdouble)
     x = lu.solve(np.asarray(b, dtype=np.float64)).astype(np.longdouble)
     for _ in range(steps):
-        Ax = np.add.reduceat(data * x[idx], ptr[:-1])
-        r = bl - Ax
+        r = bl - matvec_ld(x)
         x = x + lu.solve(r.astype(np.float64)).astype(np.longdouble)
     return x
 
@@ -71,7 +83,8 @@ def test_adjoint_formula_matches_finite_differences():
 
     def loss(th, bb):
         A, _, _ = dense_laplacian(dims, softplus(th), 1e-2)
-        x = refined_solve(A, bb)
+        w = softplus_ld(th)
+        x = refined_solve(A, lambda y: laplacian_matvec_ld(u, v, w, np.longdouble(1e-2), y), bb)
         return np.sum((x - xs) ** 2), np.asarray(x, dtype=np.float64), A
 
     L0, x, A = loss(theta, b)
@@ -81,11 +94,13 @@ def test_adjoint_formula_matches_finite_differences():
     h = 1e-5
     for k in range(5):
         d = rng.standard_normal(u.size)
+        d = d.astype(np.longdouble)
         fd = float((loss(theta + h * d, b)[0] - loss(theta - h * d, b)[0]) / (2 * h))
-        assert abs(fd - gtheta @ d) <= 1e-6 * abs(fd)
+        assert abs(fd - gtheta @ d.astype(np.float64)) <= 1e-6 * abs(fd)
         db = rng.standard_normal(n)
+        db = db.astype(np.longdouble)
         fd = float((loss(theta, b + h * db)[0] - loss(theta, b - h * db)[0]) / (2 * h))
-        assert abs(fd - lam @ db) <= 1e-6 * abs(fd)
+        assert abs(fd - lam @ db.astype(np.float64)) <= 1e-6 * abs(fd)
 
 
 @pytest.mark.gpu
@@ -114,9 +129,12 @@ def test_gpu_gradient_matches_fp64_finite_differences():
     gth = theta.grad.cpu().numpy()
     gb = b.grad.cpu().numpy()
 
+    ue, ve = F.grid_edges(*dims)
+
     def ref_loss(th, bb):
         A = F.grid_laplacian(*dims, softplus(th), 1e-2)
-        xx = refined_solve(A, bb)
+        w = softplus_ld(th)
+        xx = refined_solve(A, lambda y: laplacian_matvec_ld(ue, ve, w, np.longdouble(1e-2), y), bb)
         return np.sum((xx - xs) ** 2)
 
     assert abs(loss.item() - float(ref_loss(theta0, b0))) <= 1e-8 * float(ref_loss(theta0, b0))
@@ -124,11 +142,13 @@ def test_gpu_gradient_matches_fp64_finite_differences():
     drng = np.random.default_rng(20)
     for k in range(20):
         d = drng.standard_normal(prob.m)
-        fd = float((ref_loss(theta0 + h * d, b0) - ref_loss(theta0 - h * d, b0)) / (2 * h))
+        dl = d.astype(np.longdouble)
+        fd = float((ref_loss(theta0 + h * dl, b0) - ref_loss(theta0 - h * dl, b0)) / (2 * h))
         assert abs(gth @ d - fd) <= 1e-6 * abs(fd), (k, gth @ d, fd)
     for k in range(3):
         d = drng.standard_normal(n)
-        fd = float((ref_loss(theta0, b0 + h * d) - ref_loss(theta0, b0 - h * d)) / (2 * h))
+        dl = d.astype(np.longdouble)
+        fd = float((ref_loss(theta0, b0 + h * dl) - ref_loss(theta0, b0 - h * dl)) / (2 * h))
         assert abs(gb @ d - fd) <= 1e-6 * abs(fd)
 
 
