@@ -289,6 +289,22 @@ void launch_spmv_res_t(rchol_amd_ctx *c, const double *x, const double *b, doubl
                      (int)c->n, c->arp.p, c->aci.p, c->av.p, x, b, y, c->partials.p, c->sc);
 }
 
+template <int L>
+void launch_resid_replace_t(rchol_amd_ctx *c) {
+  hipLaunchKernelGGL((k_resid_replace<L>), dim3(c->spmv_grid()), dim3(kBlock), 0, c->stream,
+                     (int)c->n, c->arp.p, c->aci.p, c->av.p, (const double *)c->x.p,
+                     (const double *)c->b.p, c->r.p, c->partials.p, c->sc);
+}
+
+void launch_resid_replace(rchol_amd_ctx *c) {
+  switch (c->spmv_lanes) {
+    case 4: launch_resid_replace_t<4>(c); break;
+    case 16: launch_resid_replace_t<16>(c); break;
+    case 32: launch_resid_replace_t<32>(c); break;
+    default: launch_resid_replace_t<8>(c); break;
+  }
+}
+
 void launch_spmv_res(rchol_amd_ctx *c, const double *x, const double *b, double *y) {
   switch (c->spmv_lanes) {
     case 4: launch_spmv_res_t<4>(c, x, b, y); break;
@@ -378,6 +394,7 @@ int rchol_amd_ctx::launch_iteration(int parity, int slot) {
                        (const double *)p_new, (const double *)q.p, x.p, r.p, y64.p, partials.p,
                        sc);
   }
+  if (resid_period > 0) launch_resid_replace(this);
   if (ev) (void)hipEventRecord(ev[4], stream);
   return RCHOL_AMD_OK;
 }
@@ -398,6 +415,11 @@ int rchol_amd_ctx::pcg(const double *d_b, double tol, int maxit, double *d_x_out
   HIP_TRY(hipMemcpyAsync(&sc->tol, &hsc->tol, sizeof(double), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(&sc->maxit, &hsc->maxit, sizeof(int), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(&sc->zero, &hsc->zero, sizeof(int), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(&sc->resid_period, &hsc->resid_period, sizeof(int),
+                         hipMemcpyHostToDevice, stream));
+  // the captured replacement kernel reads b from the context's own buffer
+  if (resid_period > 0 && d_b != b.p)
+    HIP_TRY(hipMemcpyAsync(b.p, d_b, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
   HIP_TRY(hipEventRecord(ev0, stream));
   if (gprec == RCHOL_AMD_FP32)
     hipLaunchKernelGGL((k_init<float>), dim3(vec_grid()), dim3(kBlock), 0, stream, (int)n, d_b,

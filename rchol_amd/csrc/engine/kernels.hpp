@@ -221,6 +221,40 @@ __global__ __launch_bounds__(256) void k_update(int n, const double *__restrict_
   }
 }
 
+// True-residual replacement for the FP32-factor variant: every
+// resid_period iterations r = b - A x in FP64 replaces the recursively
+// updated residual, and the loop test is re-evaluated on it.
+template <int L>
+__global__ __launch_bounds__(256) void k_resid_replace(int n, const int *__restrict__ rp,
+                                                       const int *__restrict__ ci,
+                                                       const double *__restrict__ av,
+                                                       const double *__restrict__ x,
+                                                       const double *__restrict__ bvec,
+                                                       double *__restrict__ r, double *partials,
+                                                       DevScalars *sc) {
+  if (sc->done || sc->resid_period <= 0 || (sc->k % sc->resid_period) != 0) return;
+  constexpr int R = 64 / L;
+  const int lane = threadIdx.x & 63, grp = lane / L, sub = lane % L;
+  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int nw = gridDim.x * (blockDim.x >> 6);
+  double arr = 0.0;
+  for (int row = gw * R + grp; row < n; row += nw * R) {
+    double acc = 0.0;
+    for (int e = rp[row] + sub; e < rp[row + 1]; e += L) acc += av[e] * x[ci[e]];
+    acc = group_sum<L>(acc);
+    if (sub == 0) {
+      const double ri = bvec[row] - acc;
+      r[row] = ri;
+      arr += ri * ri;
+    }
+  }
+  double v[1] = {arr}, tot[1];
+  if (grid_reduce<1>(v, partials, &sc->ctr[CTR_GEN], tot)) {
+    sc->rr = tot[0];
+    sc->done = !(sqrt(tot[0]) > sqrt(sc->bb) * sc->tol && sc->k < sc->maxit);
+  }
+}
+
 // q = A x - b and ||q||^2 -> relres (pcg.cpp:116-118).  With bvec == null it
 // is a plain y = A x.
 template <int L>

@@ -203,3 +203,15 @@ def test_batch_columns_stop_independently():
     assert it[2] == 0 and np.all(X[:, 2] == 0)
     assert abs(int(it[0]) - i0) <= 1 and abs(int(it[1]) - i0) <= 1
     assert rel(X[:, 0], x0) <= 1e-9
+
+
+@pytest.mark.parametrize("period", [5, 50])
+def test_fp32_factor_with_residual_replacement(period):
+    A, Gs, d = load("lap3d_12_nd8")
+    print(f"case resid_replace period={period}", flush=True)
+    with rchol_amd.Solver(A, Gs, precision=rchol_amd.FP32) as s:
+        s.set_option(rchol_amd.solver.OPT_RESID_REPLACE, period)
+        x, relres, itr = s.pcg(d["b"], 1e-10, 5000)
+    true = np.linalg.norm(A @ x - d["b"]) / np.linalg.norm(d["b"])
+    assert true <= 1e-9 and relres == pytest.approx(true, rel=1e-6)
+    assert itr <= int(d["itr"]) + 5
