@@ -126,6 +126,16 @@ int rchol_amd_pcg_batch_device(rchol_amd_ctx *ctx, int k, const double *d_B, dou
                                int maxit, double *d_X, int x_in_is_initial, double *relres,
                                int *itr);
 
+/* Uneven batches: A and G are the block-diagonal stacks of nseg
+ * independent systems (system s owns rows off[s] .. off[s+1]-1,
+ * nseg <= 256).  rchol_amd_pcg_segments then runs one PCG per system
+ * (pcg.cpp:57-127 semantics each, own scalars and stopping test) with every
+ * stage of all systems in one launch; relres[nseg], itr[nseg].  No reference
+ * counterpart (batched use of pcg::iteration). */
+int rchol_amd_set_segments(rchol_amd_ctx *ctx, size_t nseg, const size_t *off);
+int rchol_amd_pcg_segments(rchol_amd_ctx *ctx, const double *b, double tol, int maxit,
+                           double *x, int x_in_is_initial, double *relres, int *itr);
+
 /* Z = G^-T G^-1 R for k interleaved columns (pcg.cpp:141-159 per column). */
 int rchol_amd_precond_batch(rchol_amd_ctx *ctx, int k, const double *R, double *Z);
 

@@ -66,7 +66,9 @@ int rchol_laplace_3d(int side, size_t *n, size_t **rowPtr, size_t **colIdx, doub
 
 /* 7-point SDDM on an nx*ny*nz grid with face conductivities exp(g),
  * g ~ N(0, sigma^2) i.i.d. (seeded), Dirichlet boundary faces with their own
- * conductivities, plus shift*I.  Index x + y*nx + z*nx*ny; rows sorted. */
+ * conductivities, plus shift*I.  Index x + y*nx + z*nx*ny; rows sorted.
+ * An axis of extent 1 has no boundary faces (nz = 1 gives the 2D 5-point
+ * operator). */
 int rchol_varcoef_3d(int nx, int ny, int nz, double sigma, double shift, unsigned long long seed,
                      size_t *n, size_t **rowPtr, size_t **colIdx, double **val);
 

@@ -20,6 +20,7 @@
 #include "../../../include/rchol_amd.h"
 #include "kernels.hpp"
 #include "kernels_mr.hpp"
+#include "kernels_seg.hpp"
 
 using namespace rchol_amd;
 
@@ -172,8 +173,26 @@ struct rchol_amd_ctx {
       graph = nullptr;
     }
   } batch;
+  // uneven batch of stacked systems
+  struct Segments {
+    int S = 0, nblocks = 0;
+    std::vector<size_t> off;
+    DevBuf<int> seg, r0, r1, sb;
+    SegScalars *ss = nullptr, *hss = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    void drop_graph() {
+      if (gexec) (void)hipGraphExecDestroy(gexec);
+      if (graph) (void)hipGraphDestroy(graph);
+      gexec = nullptr;
+      graph = nullptr;
+    }
+  } segs;
 
   ~rchol_amd_ctx() {
+    segs.drop_graph();
+    if (segs.ss) (void)hipFree(segs.ss);
+    if (segs.hss) (void)hipHostFree(segs.hss);
     batch.drop_graph();
     if (batch.bs) (void)hipFree(batch.bs);
     if (batch.hbs) (void)hipHostFree(batch.hbs);
@@ -193,6 +212,7 @@ struct rchol_amd_ctx {
     gexec = nullptr;
     graph = nullptr;
     batch.drop_graph();
+    segs.drop_graph();
   }
 
   int vec_grid() const {
@@ -221,6 +241,10 @@ struct rchol_amd_ctx {
   void launch_iteration_mr(int parity, int slot);
   template <int K>
   void launch_sweeps_mr(const double *rhs, bool with_dot, const int *stop, hipEvent_t *ev);
+  // uneven batch
+  int pcg_segments(const double *d_b, double tol, int maxit, double *d_x, int keep_x,
+                   double *relres, int *itr);
+  void launch_iteration_seg(int parity, int slot);
 };
 
 namespace {
@@ -367,7 +391,7 @@ int rchol_amd_ctx::ensure_work() {
   HIP_TRY(z64.alloc(n));
   HIP_TRY(y32.alloc(n));
   HIP_TRY(z32.alloc(n));
-  HIP_TRY(partials.alloc((size_t)ncu * 8 * 4 + 64));
+  HIP_TRY(partials.alloc(std::max<size_t>((size_t)ncu * 8 * 4 + 64, (size_t)segs.nblocks * 2 + 64)));
   nwork = n;
   return RCHOL_AMD_OK;
 }
@@ -815,6 +839,166 @@ static int pcg_batch_dev(rchol_amd_ctx *c, int K, const double *d_b, int k, doub
 
 static int precond_batch_dev(rchol_amd_ctx *c, int K, const double *d_r, double *d_z) {
   RCHOL_BATCH_DISPATCH(K, template precond_batch<KK>(d_r, d_z));
+}
+
+
+// ---------------------------------------------------------------------------
+// Uneven batch: stacked independent systems with per-system scalars
+// ---------------------------------------------------------------------------
+namespace {
+template <int L>
+void launch_spmv_p_seg_t(rchol_amd_ctx *c, SegLayout lay, const void *z, bool z32,
+                         const double *p_old, double *p_new) {
+  if (z32)
+    hipLaunchKernelGGL((k_spmv_p_seg<float, L>), dim3(c->segs.nblocks), dim3(kBlock), 0,
+                       c->stream, lay, c->arp.p, c->aci.p, c->av.p, (const float *)z, p_old, p_new,
+                       c->q.p, (const double *)c->r.p, c->partials.p, c->segs.ss);
+  else
+    hipLaunchKernelGGL((k_spmv_p_seg<double, L>), dim3(c->segs.nblocks), dim3(kBlock), 0,
+                       c->stream, lay, c->arp.p, c->aci.p, c->av.p, (const double *)z, p_old,
+                       p_new, c->q.p, (const double *)c->r.p, c->partials.p, c->segs.ss);
+}
+SegLayout seg_layout(rchol_amd_ctx *c) {
+  return SegLayout{c->segs.seg.p, c->segs.r0.p, c->segs.r1.p, c->segs.sb.p};
+}
+}  // namespace
+
+void rchol_amd_ctx::launch_iteration_seg(int parity, int slot) {
+  double *p_old = parity ? p1.p : p0.p;
+  double *p_new = parity ? p0.p : p1.p;
+  const int *stop = &segs.ss->all_done;
+  hipEvent_t *ev = profile ? &pev[(size_t)slot * 5] : nullptr;
+  const SegLayout lay = seg_layout(this);
+  const bool f32 = gprec == RCHOL_AMD_FP32;
+  if (ev) (void)hipEventRecord(ev[0], stream);
+  if (f32)
+    dispatch_sweeps<float, float>(this, r.p, y32.p, z32.p, stop, false, nullptr, ev);
+  else
+    dispatch_sweeps<double, double>(this, r.p, y64.p, z64.p, stop, false, nullptr, ev);
+  const dim3 g(segs.nblocks), blk(kBlock);
+  if (f32)
+    hipLaunchKernelGGL((k_dot_seg<float>), g, blk, 0, stream, lay, (const double *)r.p,
+                       (const float *)z32.p, partials.p, segs.ss);
+  else
+    hipLaunchKernelGGL((k_dot_seg<double>), g, blk, 0, stream, lay, (const double *)r.p,
+                       (const double *)z64.p, partials.p, segs.ss);
+  const void *z = f32 ? (const void *)z32.p : (const void *)z64.p;
+  switch (spmv_lanes) {
+    case 4: launch_spmv_p_seg_t<4>(this, lay, z, f32, p_old, p_new); break;
+    case 16: launch_spmv_p_seg_t<16>(this, lay, z, f32, p_old, p_new); break;
+    case 32: launch_spmv_p_seg_t<32>(this, lay, z, f32, p_old, p_new); break;
+    default: launch_spmv_p_seg_t<8>(this, lay, z, f32, p_old, p_new); break;
+  }
+  if (ev) (void)hipEventRecord(ev[3], stream);
+  if (f32)
+    hipLaunchKernelGGL((k_update_seg<float>), g, blk, 0, stream, lay, (const double *)p_new,
+                       (const double *)q.p, x.p, r.p, y32.p, partials.p, segs.ss);
+  else
+    hipLaunchKernelGGL((k_update_seg<double>), g, blk, 0, stream, lay, (const double *)p_new,
+                       (const double *)q.p, x.p, r.p, y64.p, partials.p, segs.ss);
+  if (ev) (void)hipEventRecord(ev[4], stream);
+}
+
+int rchol_amd_ctx::pcg_segments(const double *d_b, double tol, int maxit, double *d_x, int keep_x,
+                                double *relres, int *itr) {
+  if (!hasA || !hasG) return fail(RCHOL_AMD_ESTATE, "A and G must be uploaded before pcg");
+  if (segs.S == 0 || segs.off.back() != n)
+    return fail(RCHOL_AMD_ESTATE, "segments must be set for the uploaded A");
+  int rc = ensure_work();
+  if (rc) return rc;
+  if (keep_x && d_x != x.p)
+    HIP_TRY(hipMemcpyAsync(x.p, d_x, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  segs.hss->tol = tol;
+  segs.hss->maxit = maxit;
+  segs.hss->nseg = segs.S;
+  HIP_TRY(hipMemcpyAsync(&segs.ss->tol, &segs.hss->tol, sizeof(double), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(&segs.ss->maxit, &segs.hss->maxit, sizeof(int), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(&segs.ss->nseg, &segs.hss->nseg, sizeof(int), hipMemcpyHostToDevice, stream));
+  hsc->err = 0;
+  HIP_TRY(hipMemcpyAsync(&sc->err, &hsc->err, sizeof(int), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipEventRecord(ev0, stream));
+  const SegLayout lay = seg_layout(this);
+  if (gprec == RCHOL_AMD_FP32)
+    hipLaunchKernelGGL((k_init_seg<float>), dim3(segs.nblocks), dim3(kBlock), 0, stream, lay, d_b,
+                       r.p, x.p, keep_x, y32.p, partials.p, segs.ss);
+  else
+    hipLaunchKernelGGL((k_init_seg<double>), dim3(segs.nblocks), dim3(kBlock), 0, stream, lay,
+                       d_b, r.p, x.p, keep_x, y64.p, partials.p, segs.ss);
+  HIP_TRY(hipGetLastError());
+  int G = graph_iters > 0 ? graph_iters : 8;
+  if (G & 1) G++;
+  if (profile && pev.size() < (size_t)G * 5) {
+    for (hipEvent_t ev : pev) (void)hipEventDestroy(ev);
+    pev.assign((size_t)G * 5, nullptr);
+    for (auto &ev : pev) HIP_TRY(hipEventCreate(&ev));
+    drop_graph();
+  }
+  for (int i = 0; i < 4; i++) stats.kernel_ms[i] = 0.0;
+  stats.kernel_iters = 0;
+  if (graph_iters > 0 && !segs.gexec && !(profile && profile_no_graph)) {
+    HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < G; i++) launch_iteration_seg(i & 1, i);
+    hipError_t e = hipStreamEndCapture(stream, &segs.graph);
+    if (e == hipSuccess) e = hipGraphInstantiate(&segs.gexec, segs.graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      segs.drop_graph();
+      if (!profile) return fail(RCHOL_AMD_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+      profile_no_graph = 1;
+    }
+  }
+  int maxk_seen = 0;
+  for (;;) {
+    if (segs.gexec) {
+      HIP_TRY(hipGraphLaunch(segs.gexec, stream));
+    } else {
+      for (int i = 0; i < G; i++) launch_iteration_seg(i & 1, i);
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemcpyAsync(segs.hss, segs.ss, sizeof(SegScalars), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(&hsc->err, &sc->err, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    int maxk = 0;
+    for (int s = 0; s < segs.S; s++) maxk = std::max(maxk, segs.hss->k[s]);
+    if (profile) {
+      const int ran = std::min(G, maxk - maxk_seen);
+      for (int i = 0; i < ran; i++)
+        for (int kk = 0; kk < 4; kk++) {
+          float ms = 0.f;
+          HIP_TRY(hipEventElapsedTime(&ms, pev[(size_t)i * 5 + kk], pev[(size_t)i * 5 + kk + 1]));
+          stats.kernel_ms[kk] += ms;
+        }
+      stats.kernel_iters += ran;
+    }
+    maxk_seen = maxk;
+    if (hsc->err || segs.hss->all_done) break;
+  }
+  if (hsc->err)
+    return fail(RCHOL_AMD_ESTALL, "triangular sweep wait exceeded its bound (err=%d)", hsc->err);
+  switch (spmv_lanes) {
+    case 4: hipLaunchKernelGGL((k_spmv_res_seg<4>), dim3(segs.nblocks), dim3(kBlock), 0, stream, lay, arp.p, aci.p, av.p, (const double *)x.p, d_b, tmp64a.p, partials.p, segs.ss); break;
+    case 16: hipLaunchKernelGGL((k_spmv_res_seg<16>), dim3(segs.nblocks), dim3(kBlock), 0, stream, lay, arp.p, aci.p, av.p, (const double *)x.p, d_b, tmp64a.p, partials.p, segs.ss); break;
+    case 32: hipLaunchKernelGGL((k_spmv_res_seg<32>), dim3(segs.nblocks), dim3(kBlock), 0, stream, lay, arp.p, aci.p, av.p, (const double *)x.p, d_b, tmp64a.p, partials.p, segs.ss); break;
+    default: hipLaunchKernelGGL((k_spmv_res_seg<8>), dim3(segs.nblocks), dim3(kBlock), 0, stream, lay, arp.p, aci.p, av.p, (const double *)x.p, d_b, tmp64a.p, partials.p, segs.ss); break;
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ev1, stream));
+  HIP_TRY(hipMemcpyAsync(segs.hss, segs.ss, sizeof(SegScalars), hipMemcpyDeviceToHost, stream));
+  if (d_x != x.p)
+    HIP_TRY(hipMemcpyAsync(d_x, x.p, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+  int maxk = 0;
+  for (int s = 0; s < segs.S; s++) {
+    relres[s] = segs.hss->relres[s];
+    itr[s] = segs.hss->k[s];
+    maxk = std::max(maxk, itr[s]);
+  }
+  stats.last_iters = maxk;
+  stats.last_relres = relres[0];
+  stats.last_solve_ms = ms;
+  return RCHOL_AMD_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1308,6 +1492,69 @@ int rchol_amd_precond_batch(rchol_amd_ctx *c, int k, const double *R, double *Z)
   HIP_TRY(hipMemcpy(h.data(), c->batch.res.p, m * sizeof(double), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < n; i++)
     for (int j = 0; j < k; j++) Z[P[i] * k + j] = h[i * K + j];
+  return RCHOL_AMD_OK;
+}
+
+
+int rchol_amd_set_segments(rchol_amd_ctx *c, size_t nseg, const size_t *off) {
+  if (!c || !off) return fail(RCHOL_AMD_EINVAL, "null argument");
+  if (nseg < 1 || nseg > (size_t)kMaxSeg)
+    return fail(RCHOL_AMD_EINVAL, "segment count %zu outside [1, %d]", nseg, kMaxSeg);
+  if (!c->hasA) return fail(RCHOL_AMD_ESTATE, "A must be uploaded before the segments");
+  if (off[0] != 0 || off[nseg] != c->n) return fail(RCHOL_AMD_EINVAL, "segments must cover 0..n");
+  for (size_t s = 0; s < nseg; s++)
+    if (off[s + 1] <= off[s]) return fail(RCHOL_AMD_EINVAL, "segment %zu is empty", s);
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->segs.drop_graph();
+  // segment-major block layout, ~ncu*8 blocks in proportion to rows
+  const size_t target = (size_t)c->ncu * 8;
+  std::vector<int> seg, r0, r1, sb(1, 0);
+  for (size_t s = 0; s < nseg; s++) {
+    const size_t rows = off[s + 1] - off[s];
+    size_t nb = std::max<size_t>(1, (size_t)std::llround((double)target * rows / c->n));
+    nb = std::min(nb, (rows + 255) / 256);
+    nb = std::max<size_t>(nb, 1);
+    for (size_t b = 0; b < nb; b++) {
+      seg.push_back((int)s);
+      r0.push_back((int)(off[s] + rows * b / nb));
+      r1.push_back((int)(off[s] + rows * (b + 1) / nb));
+    }
+    sb.push_back((int)seg.size());
+  }
+  HIP_TRY(c->segs.seg.upload(seg.data(), seg.size()));
+  HIP_TRY(c->segs.r0.upload(r0.data(), r0.size()));
+  HIP_TRY(c->segs.r1.upload(r1.data(), r1.size()));
+  HIP_TRY(c->segs.sb.upload(sb.data(), sb.size()));
+  if (!c->segs.ss) {
+    HIP_TRY(hipMalloc((void **)&c->segs.ss, sizeof(SegScalars)));
+    HIP_TRY(hipHostMalloc((void **)&c->segs.hss, sizeof(SegScalars), hipHostMallocDefault));
+  }
+  HIP_TRY(hipMemset(c->segs.ss, 0, sizeof(SegScalars)));
+  memset(c->segs.hss, 0, sizeof(SegScalars));
+  c->segs.S = (int)nseg;
+  c->segs.nblocks = (int)seg.size();
+  c->segs.off.assign(off, off + nseg + 1);
+  // partial sums: 2 per block
+  if (c->partials.n < (size_t)c->segs.nblocks * 2 + 64)
+    HIP_TRY(c->partials.alloc((size_t)c->segs.nblocks * 2 + 64));
+  return RCHOL_AMD_OK;
+}
+
+int rchol_amd_pcg_segments(rchol_amd_ctx *c, const double *b, double tol, int maxit, double *x,
+                           int x_in_is_initial, double *relres, int *itr) {
+  if (!c || !b || !x || !relres || !itr) return fail(RCHOL_AMD_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = c->ensure_work();
+  if (rc) return rc;
+  if (c->partials.n < (size_t)c->segs.nblocks * 2 + 64)
+    HIP_TRY(c->partials.alloc((size_t)c->segs.nblocks * 2 + 64));
+  HIP_TRY(hipMemcpyAsync(c->b.p, b, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (x_in_is_initial)
+    HIP_TRY(hipMemcpyAsync(c->x.p, x, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  rc = c->pcg_segments(c->b.p, tol, maxit, c->x.p, x_in_is_initial, relres, itr);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(x, c->x.p, c->n * sizeof(double), hipMemcpyDeviceToHost));
   return RCHOL_AMD_OK;
 }
 

@@ -91,13 +91,16 @@ int rchol_varcoef_3d(int nx, int ny, int nz, double sigma, double shift, unsigne
       }
     }
   }
-  // Dirichlet boundary faces: conductivity to the outside value 0
-  for (int a = 0; a < 3; a++)
+  // Dirichlet boundary faces: conductivity to the outside value 0 (an axis
+  // of extent 1 is a collapsed dimension -- 2D problems -- and has none)
+  for (int a = 0; a < 3; a++) {
+    if (ext[a] == 1) continue;
     for (size_t i = 0; i < N; i++) {
       const int c = (int)((i / st[a]) % ext[a]);
       if (c == 0) diag[i] += std::exp(g(gen));
       if (c == ext[a] - 1) diag[i] += std::exp(g(gen));
     }
+  }
   return assemble7(nx, ny, nz, w, diag, n, rowPtr, colIdx, val);
 }
 

@@ -136,6 +136,29 @@ class Solver:
         if self.n is None:
             self.n = n
 
+    def set_segments(self, off):
+        """Uneven batch: A and G are block-diagonal stacks, system s owns rows
+        off[s]..off[s+1]-1."""
+        o, po = _u64(off)
+        _lib.check(self._lib.rchol_amd_set_segments(self._h, o.size - 1, po))
+        self.nseg = o.size - 1
+
+    def pcg_segments(self, b, tol, maxit, x0=None):
+        """One PCG per stacked system -> (x, relres[nseg], itr[nseg])."""
+        b, pb = _f64(b)
+        keep = 0
+        x = np.zeros_like(b)
+        if x0 is not None:
+            x[:] = x0
+            keep = 1
+        rr = np.zeros(self.nseg)
+        it = np.zeros(self.nseg, dtype=np.int32)
+        _lib.check(self._lib.rchol_amd_pcg_segments(self._h, pb, float(tol), int(maxit),
+                                                    x.ctypes.data_as(_lib.f64p), keep,
+                                                    rr.ctypes.data_as(_lib.f64p),
+                                                    it.ctypes.data_as(_lib.i32p)))
+        return x, rr, it
+
     def set_perm(self, perm):
         """Vectors of later solves are in the original numbering; A and G
         factor A(perm, perm) (perm=None clears)."""
@@ -235,6 +258,26 @@ class Solver:
         s = Stats()
         _lib.check(self._lib.rchol_amd_get_stats(self._h, ctypes.byref(s)))
         return s.as_dict()
+
+
+def stack_systems(systems):
+    """Block-diagonal stack of [(A_s, (colPtr, rowIdx, val)_s)] -> (A, G, off)."""
+    As, cps, ris, vs, off = [], [], [0], [], [0]
+    cps = [np.zeros(1, dtype=np.uint64)]
+    ris, vs = [], []
+    base_n, base_nnz = 0, 0
+    for A, (cp, ri, v) in systems:
+        As.append(sp.csr_matrix(A))
+        cp = np.asarray(cp, dtype=np.uint64)
+        cps.append(cp[1:] + np.uint64(base_nnz))
+        ris.append(np.asarray(ri, dtype=np.uint64) + np.uint64(base_n))
+        vs.append(np.asarray(v, dtype=np.float64))
+        base_n += A.shape[0]
+        base_nnz += int(cp[-1])
+        off.append(base_n)
+    A = sp.block_diag(As, format="csr")
+    G = (np.concatenate(cps), np.concatenate(ris), np.concatenate(vs))
+    return A, G, np.asarray(off, dtype=np.uint64)
 
 
 def pcg(A, b, epsilon, maxitr, G, Gt=None):

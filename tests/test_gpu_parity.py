@@ -215,3 +215,34 @@ def test_fp32_factor_with_residual_replacement(period):
     true = np.linalg.norm(A @ x - d["b"]) / np.linalg.norm(d["b"])
     assert true <= 1e-9 and relres == pytest.approx(true, rel=1e-6)
     assert itr <= int(d["itr"]) + 5
+
+
+@pytest.mark.parametrize("precision", [rchol_amd.FP64, rchol_amd.FP32])
+def test_uneven_batch_matches_per_system_oracle(precision):
+    from rchol_amd import factor as F
+    print(f"case segments prec={precision}", flush=True)
+    rng = np.random.default_rng(48)
+    systems = []
+    for s, side in enumerate([6, 13, 9, 20, 7]):
+        A = F.varcoef_3d(side, side + 1, 1, 1.0, 0.0, 100 + s)
+        p, bl = F.nd_grid(side, side + 1, 1, 2)
+        Gs, po, _ = F.factor(A, p, bl, "mis", s + 1)
+        P = po.astype(np.int64)
+        systems.append((A[P][:, P].tocsr(), Gs))
+    Au, Gu, off = rchol_amd.stack_systems(systems)
+    b = rng.uniform(-1, 1, Au.shape[0])
+    with rchol_amd.Solver(Au, Gu, precision=precision) as s:
+        s.set_segments(off)
+        x, rr, it = s.pcg_segments(b, 1e-10, 2000)
+        x2, rr2, it2 = s.pcg_segments(b, 1e-10, 2000)  # graph replay, same result
+    np.testing.assert_array_equal(x, x2)
+    for k, (A, Gs) in enumerate(systems):
+        lo, hi = int(off[k]), int(off[k + 1])
+        xo, ro, io = O.pcg(A, b[lo:hi], 1e-10, 2000, Gs)
+        true = np.linalg.norm(A @ x[lo:hi] - b[lo:hi]) / np.linalg.norm(b[lo:hi])
+        assert true <= 1e-9 and rr[k] == pytest.approx(true, rel=1e-6)
+        if precision == rchol_amd.FP64:
+            assert abs(int(it[k]) - io) <= 1
+            assert rel(x[lo:hi], xo) <= 1e-8
+        else:
+            assert int(it[k]) <= io + 5
