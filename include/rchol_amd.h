@@ -164,6 +164,13 @@ int rchol_amd_edge_grad_device(rchol_amd_ctx *ctx, size_t m, const long long *d_
                                const long long *d_v, const double *d_lam, const double *d_x,
                                double *d_g);
 
+/* Autotuner: times the preconditioner application and the SpMV with 4, 8
+ * and 16 lanes per row on the uploaded matrices (median of reps runs) and
+ * keeps the fastest (RCHOL_AMD_OPT_TRSV_LANES / _SPMV_LANES).  Outputs may
+ * be NULL.  No reference counterpart. */
+int rchol_amd_autotune(rchol_amd_ctx *ctx, int reps, int *trsv_lanes, int *spmv_lanes,
+                       double *trsv_ms, double *spmv_ms);
+
 int rchol_amd_set_option(rchol_amd_ctx *ctx, int key, double value);
 int rchol_amd_get_stats(rchol_amd_ctx *ctx, rchol_amd_stats *out);
 

@@ -1558,6 +1558,58 @@ int rchol_amd_pcg_segments(rchol_amd_ctx *c, const double *b, double tol, int ma
   return RCHOL_AMD_OK;
 }
 
+
+int rchol_amd_autotune(rchol_amd_ctx *c, int reps, int *trsv_lanes, int *spmv_lanes,
+                       double *trsv_ms, double *spmv_ms) {
+  if (!c) return fail(RCHOL_AMD_EINVAL, "null context");
+  if (!c->hasA || !c->hasG) return fail(RCHOL_AMD_ESTATE, "A and G must be uploaded before tuning");
+  if (reps < 1) reps = 3;
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = c->ensure_work();
+  if (rc) return rc;
+  const size_t n = c->n;
+  std::vector<double> h(n);
+  for (size_t i = 0; i < n; i++) h[i] = 1.0 + (double)((i * 2654435761u) % 1000) * 1e-3;
+  HIP_TRY(hipMemcpy(c->b.p, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  const int cand[3] = {4, 8, 16};
+  auto time_ms = [&](auto fn) -> double {
+    std::vector<float> t;
+    for (int r = 0; r <= reps; r++) {  // first run warms up
+      (void)hipEventRecord(c->ev0, c->stream);
+      fn();
+      (void)hipEventRecord(c->ev1, c->stream);
+      (void)hipStreamSynchronize(c->stream);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+      if (r) t.push_back(ms);
+    }
+    std::sort(t.begin(), t.end());
+    return t[t.size() / 2];
+  };
+  int best_t = c->trsv_lanes, best_s = c->spmv_lanes;
+  double bt = 1e30, bs = 1e30;
+  for (int L : cand) {
+    c->trsv_lanes = L;
+    rc = c->size_trsv_grid();
+    if (rc) return rc;
+    double ms = time_ms([&]() { (void)c->precond(c->b.p, c->tmp64a.p); });
+    if (ms < bt) { bt = ms; best_t = L; }
+    c->spmv_lanes = L;
+    ms = time_ms([&]() { launch_spmv_res(c, c->b.p, nullptr, c->q.p); });
+    if (ms < bs) { bs = ms; best_s = L; }
+  }
+  c->trsv_lanes = best_t;
+  c->spmv_lanes = best_s;
+  rc = c->size_trsv_grid();
+  if (rc) return rc;
+  c->drop_graph();
+  if (trsv_lanes) *trsv_lanes = best_t;
+  if (spmv_lanes) *spmv_lanes = best_s;
+  if (trsv_ms) *trsv_ms = bt;
+  if (spmv_ms) *spmv_ms = bs;
+  return RCHOL_AMD_OK;
+}
+
 int rchol_amd_pcg_csr(size_t n, const size_t *aRowPtr, const size_t *aColIdx, const double *aVal,
                       const double *b, double tol, int maxit, const size_t *gColPtr,
                       const size_t *gRowIdx, const double *gVal, double *x, int x_in_is_initial,

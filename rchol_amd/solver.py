@@ -136,6 +136,16 @@ class Solver:
         if self.n is None:
             self.n = n
 
+    def autotune(self, reps=3):
+        """Pick lanes-per-row of the sweeps and the SpMV by timing them."""
+        tl, sl = ctypes.c_int(), ctypes.c_int()
+        tm, sm = ctypes.c_double(), ctypes.c_double()
+        _lib.check(self._lib.rchol_amd_autotune(self._h, int(reps), ctypes.byref(tl),
+                                                ctypes.byref(sl), ctypes.byref(tm),
+                                                ctypes.byref(sm)))
+        return {"trsv_lanes": tl.value, "spmv_lanes": sl.value, "precond_ms": tm.value,
+                "spmv_ms": sm.value}
+
     def set_segments(self, off):
         """Uneven batch: A and G are block-diagonal stacks, system s owns rows
         off[s]..off[s+1]-1."""

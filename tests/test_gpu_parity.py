@@ -246,3 +246,15 @@ def test_uneven_batch_matches_per_system_oracle(precision):
             assert rel(x[lo:hi], xo) <= 1e-8
         else:
             assert int(it[k]) <= io + 5
+
+
+def test_autotune_keeps_results():
+    A, Gs, d = load("lap3d_12_nd8")
+    print("case autotune", flush=True)
+    with rchol_amd.Solver(A, Gs) as s:
+        t = s.autotune(reps=2)
+        assert t["trsv_lanes"] in (4, 8, 16) and t["spmv_lanes"] in (4, 8, 16)
+        z = s.precond(d["b"])
+        x, relres, itr = s.pcg(d["b"], 1e-10, 5000)
+    assert rel(z, d["z"]) <= PRECOND_RTOL
+    assert abs(itr - int(d["itr"])) <= 1
