@@ -44,7 +44,7 @@ def kernel_bytes(n, nnz_a, nnz_g, fp32):
 KERNEL_NAMES = ["sptrsv_fwd", "sptrsv_bwd", "spmv_p", "update"]
 
 
-def main():
+def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -60,19 +60,30 @@ def main():
     ap.add_argument("--graph-iters", type=int, default=16)
     ap.add_argument("--trsv-lanes", type=int, default=8)
     ap.add_argument("--spmv-lanes", type=int, default=8)
-    args = ap.parse_args()
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4a", "c4b", "c5"],
+                    help="c3 (default) is the headline; the others print their own line")
+    ap.add_argument("--nrhs", type=int, default=32)
+    ap.add_argument("--nsys", type=int, default=48)
+    return ap.parse_args()
 
+
+def main():
+    args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is design-only)", file=sys.stderr)
         return 2
-
     import rchol_amd
-    from rchol_amd import factor as F
-
     if rchol_amd.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         return 1
+    return {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4a": run_c4a, "c4b": run_c4b,
+            "c5": run_c5}[args.config](args)
+
+
+def run_c3(args):
+    import rchol_amd
+    from rchol_amd import factor as F
 
     s = args.side
     depth = int(np.log2(args.leaves)) + 1
@@ -222,9 +233,254 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(res), flush=True)
+    _emit(res)
     s32.close()
     s64.close()
+    return 0
+
+
+def _emit(res):
+    print(json.dumps(res), flush=True)
+
+
+def _base(metric, value, unit, steps, warmup, hib, config, dtype):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": value * 1e3 if unit == "s" else None,
+            "higher_is_better": hib, "scaling": "strong", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic", "config": config}
+
+
+def _timed(fn, k):
+    out, dev = None, []
+    for _ in range(k):
+        out = fn()
+        dev.append(out[-1])
+    return out, float(np.mean(dev))
+
+
+def run_c1(args):
+    """32^3 Laplacian, sequential rchol (natural order), FP64 GPU vs the oracle."""
+    import rchol_amd
+    from rchol_amd import factor as F
+    from oracle import oracle as O
+    side = 32 if args.side == 256 else args.side
+    A = F.laplace_3d(side)
+    n = A.shape[0]
+    Gs, _, _ = F.factor(A, None, [n], "static", 1)
+    b = np.random.default_rng(7).uniform(-1, 1, n)
+    with rchol_amd.Solver(A, Gs, graph_iters=args.graph_iters) as s:
+        for _ in range(args.warmup):
+            s.pcg(b, 1e-10, 5000)
+        def one():
+            x, rr, it = s.pcg(b, 1e-10, 5000)
+            return x, rr, it, s.stats()["last_solve_ms"] / 1e3
+        (x, rr, it, _), t = _timed(one, args.steps)
+        st = s.stats()
+    t0 = time.perf_counter()
+    xo, ro, io = O.pcg(A, b, 1e-10, 5000, Gs)
+    tc = time.perf_counter() - t0
+    res = _base("PCG time-to-1e-10, 32^3 Laplacian (sequential rchol)", t, "s", args.steps,
+                args.warmup, False,
+                {"workload": f"{side}^3 7-point Laplacian (laplace_3d), sequential rchol, "
+                             "natural order, seed 1, b ~ U[-1,1] seed 7, FP64",
+                 "n": n, "nnz_g": int(Gs[1].size), "fwd_levels": st["fwd_levels"]}, "f64")
+    res.update({"iters": it, "oracle_iters": io, "relres": rr, "oracle_relres": ro,
+                "x_rel_diff_vs_oracle": float(np.linalg.norm(x - xo) / np.linalg.norm(xo)),
+                "cpu_baseline": {"value": tc, "unit": "s", "cores": 1, "kind": "port",
+                                 "sample": "full oracle PCG solve (C restatement of pcg.cpp)"}})
+    _emit(res)
+    return 0
+
+
+def _solve_pair(Ap, Gs, bp, args, tag):
+    import rchol_amd
+    out = {}
+    for name, prec in (("fp64", rchol_amd.FP64), ("mixed", rchol_amd.FP32)):
+        with rchol_amd.Solver(Ap, Gs, prec, graph_iters=args.graph_iters) as s:
+            for _ in range(args.warmup):
+                s.pcg(bp, args.tol, args.maxit)
+            def one():
+                x, rr, it = s.pcg(bp, args.tol, args.maxit)
+                return x, rr, it, s.stats()["last_solve_ms"] / 1e3
+            (x, rr, it, _), t = _timed(one, args.steps)
+            st = s.stats()
+            if name == "fp64":
+                xref, _, iref = s.pcg(bp, 1e-14, 20000)
+            out[name] = {"time_s": t, "iters": it, "relres": rr, "x": x,
+                         "levels": (st["fwd_levels"], st["bwd_levels"])}
+        log(f"{tag} {name}: {it} iterations, {t * 1e3:.1f} ms")
+    for name in ("fp64", "mixed"):
+        out[name]["err_vs_fp64_1e-14"] = float(np.linalg.norm(out[name].pop("x") - xref) /
+                                               np.linalg.norm(xref))
+    out["ref_iters_1e-14"] = iref
+    return out
+
+
+def run_c2(args):
+    """128^3 Laplacian, ND 64 leaves: FP64 vs FP32 factor, both orderings."""
+    from rchol_amd import factor as F
+    side = 128 if args.side == 256 else args.side
+    A = F.laplace_3d(side)
+    n = A.shape[0]
+    b = np.random.default_rng(7).uniform(-1, 1, n)
+    perm, blocks = F.nd_grid(side, side, side, 7)
+    res = None
+    for order in ("mis", "static"):
+        Gs, po, tf = F.factor(A, perm, blocks, order, 1)
+        P = po.astype(np.int64)
+        Ap = A[P][:, P].tocsr()
+        Ap.sort_indices()
+        r = _solve_pair(Ap, Gs, b[P], args, f"c2/{order}")
+        r["fill"] = 2 * Gs[1].size / A.nnz
+        if res is None:
+            res = _base("PCG time-to-1e-10, 128^3 Laplacian ND 64 leaves; mixed vs FP64",
+                        r["mixed"]["time_s"], "s", args.steps, args.warmup, False,
+                        {"workload": f"{side}^3 laplace_3d, ND 64 leaves, rchol seed 1, "
+                                     "b ~ U[-1,1] seed 7", "n": n}, "f64 + f32 factor")
+        res[order] = r
+    _emit(res)
+    return 0
+
+
+def run_c4a(args):
+    """C3 matrix, a block of nrhs right-hand sides: per-RHS time vs FP64 single RHS."""
+    import rchol_amd
+    from rchol_amd import factor as F
+    s = args.side
+    depth = int(np.log2(args.leaves)) + 1
+    A = F.varcoef_3d(s, s, s, 1.5, 1e-3, 20261202)
+    n = A.shape[0]
+    perm, blocks = F.nd_grid(s, s, s, depth)
+    log("c4a: factor")
+    Gs, po, _ = F.factor(A, perm, blocks, args.order, 1)
+    P = po.astype(np.int64)
+    Ap = A[P][:, P].tocsr()
+    Ap.sort_indices()
+    del A
+    k = args.nrhs
+    B = np.random.default_rng(11).uniform(-1, 1, (n, k))
+    out = {}
+    for name, prec in (("fp64", rchol_amd.FP64), ("mixed", rchol_amd.FP32)):
+        with rchol_amd.Solver(Ap, Gs, prec, graph_iters=args.graph_iters) as sv:
+            x1, rr1, it1 = sv.pcg(B[:, 0].copy(), args.tol, args.maxit)
+            t1 = []
+            for _ in range(args.steps):
+                sv.pcg(B[:, 0].copy(), args.tol, args.maxit)
+                t1.append(sv.stats()["last_solve_ms"] / 1e3)
+            sv.pcg_batch(B, args.tol, args.maxit)
+            tb = []
+            for _ in range(args.steps):
+                X, rr, it = sv.pcg_batch(B, args.tol, args.maxit)
+                tb.append(sv.stats()["last_solve_ms"] / 1e3)
+            true = [float(np.linalg.norm(Ap @ X[:, c] - B[:, c]) / np.linalg.norm(B[:, c]))
+                    for c in range(k)]
+            out[name] = {"single_rhs_s": float(np.mean(t1)), "single_iters": it1,
+                         "batch_s": float(np.mean(tb)), "per_rhs_s": float(np.mean(tb)) / k,
+                         "iters_max": int(it.max()), "iters_min": int(it.min()),
+                         "max_true_relres": max(true)}
+            log(f"c4a {name}: single {out[name]['single_rhs_s']*1e3:.1f} ms, batch of {k} "
+                f"{out[name]['batch_s']*1e3:.1f} ms")
+    per_rhs = out["mixed"]["per_rhs_s"]
+    res = _base("PCG per-RHS time-to-1e-10, 32-RHS batch on the 256^3 var-coef problem",
+                per_rhs, "s", args.steps, 0, False,
+                {"workload": f"{s}^3 var-coef Poisson, {k} RHS interleaved, FP32 factor batch "
+                             "vs FP64 single RHS", "n": n, "nrhs": k}, "f64 + f32 factor")
+    res.update(out)
+    res["speedup_per_rhs_vs_fp64_single"] = out["fp64"]["single_rhs_s"] / per_rhs
+    _emit(res)
+    return 0
+
+
+def run_c4b(args):
+    """nsys uneven 2D 5-point SDDM systems in one launch per stage vs one by one."""
+    import rchol_amd
+    from rchol_amd import factor as F
+    rng = np.random.default_rng(48)
+    sides = np.exp(rng.uniform(np.log(64), np.log(1024), args.nsys)).astype(int)
+    systems, bs = [], []
+    t0 = time.perf_counter()
+    for i, sd in enumerate(sides):
+        A = F.varcoef_3d(int(sd), int(sd), 1, 1.0, 0.0, 1000 + i)
+        perm, blocks = F.nd_grid(int(sd), int(sd), 1, 4)
+        Gs, po, _ = F.factor(A, perm, blocks, "mis", i + 1)
+        P = po.astype(np.int64)
+        systems.append((A[P][:, P].tocsr(), Gs))
+        bs.append(rng.uniform(-1, 1, A.shape[0]))
+    log(f"c4b: {args.nsys} systems factored in {time.perf_counter() - t0:.1f}s, "
+        f"n total {sum(a.shape[0] for a, _ in systems)}")
+    Au, Gu, off = rchol_amd.stack_systems(systems)
+    b = np.concatenate(bs)
+    with rchol_amd.Solver(Au, Gu, rchol_amd.FP32, graph_iters=args.graph_iters) as sv:
+        tune = sv.autotune(reps=3)
+        sv.set_segments(off)
+        sv.pcg_segments(b, args.tol, args.maxit)
+        tb = []
+        for _ in range(args.steps):
+            x, rr, it = sv.pcg_segments(b, args.tol, args.maxit)
+            tb.append(sv.stats()["last_solve_ms"] / 1e3)
+    t_batch = float(np.mean(tb))
+    t_seq = 0.0
+    worst = 0.0
+    for k, (A, Gs) in enumerate(systems):
+        lo, hi = int(off[k]), int(off[k + 1])
+        worst = max(worst, float(np.linalg.norm(A @ x[lo:hi] - b[lo:hi]) /
+                                 np.linalg.norm(b[lo:hi])))
+        with rchol_amd.Solver(A, Gs, rchol_amd.FP32, graph_iters=args.graph_iters) as sv:
+            sv.pcg(bs[k], args.tol, args.maxit)
+            sv.pcg(bs[k], args.tol, args.maxit)
+            t_seq += sv.stats()["last_solve_ms"] / 1e3
+    res = _base("uneven batch of 2D 5-point SDDM systems to 1e-10: one launch per stage",
+                t_batch, "s", args.steps, 1, False,
+                {"workload": f"{args.nsys} 2D 5-point SDDM, sides log-uniform 64..1024, "
+                             "log-normal weights sigma=1, FP32 factor, ND 8 leaves + MIS",
+                 "sides": sides.tolist(), "n_total": int(off[-1])}, "f64 + f32 factor")
+    res.update({"sequential_s": t_seq, "speedup_vs_sequential": t_seq / t_batch,
+                "iters_max": int(it.max()), "iters_min": int(it.min()),
+                "max_true_relres": worst, "autotune": tune})
+    _emit(res)
+    return 0
+
+
+def run_c5(args):
+    """100^3 weighted graph Laplacian: differentiable solve, mixed forward +
+    adjoint, gradient vs an FP64 adjoint at full size."""
+    import torch
+    import rchol_amd
+    from rchol_amd import autograd as AG
+    side = 100 if args.side == 256 else args.side
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(5)
+    n = side ** 3
+    grads, times = {}, {}
+    for name, prec in (("mixed", rchol_amd.FP32), ("fp64", rchol_amd.FP64)):
+        prob = AG.GridLaplacian(side, side, side, shift=1e-2, leaves=16, precision=prec,
+                                tol=1e-10 if name == "mixed" else 1e-13)
+        theta0 = np.random.default_rng(1).standard_normal(prob.m)
+        b0 = rng.uniform(-1, 1, n) if not grads else b0
+        xs = np.random.default_rng(2).uniform(-1, 1, n)
+        theta = torch.tensor(theta0, device=dev, requires_grad=True)
+        w = torch.nn.functional.softplus(theta)
+        prob.solver(w.detach().cpu().numpy())  # factor outside the timed region
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        x = AG.solve(w, torch.tensor(b0, device=dev), prob)
+        loss = ((x - torch.tensor(xs, device=dev)) ** 2).sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        times[name] = time.perf_counter() - t
+        grads[name] = theta.grad.cpu().numpy()
+        log(f"c5 {name}: forward {prob.last['forward']}, adjoint {prob.last['adjoint']}, "
+            f"{times[name]:.3f}s")
+    g, gr = grads["mixed"], grads["fp64"]
+    res = _base("differentiable solve: forward + adjoint time, 100^3 graph Laplacian",
+                times["mixed"], "s", 1, 0, False,
+                {"workload": f"{side}^3 6-neighbour graph Laplacian, w = softplus(theta), "
+                             "+1e-2 I, loss ||x - x*||^2, mixed forward/adjoint PCG",
+                 "n": n, "edges": int(g.size)}, "f64 + f32 factor")
+    res.update({"fp64_time_s": times["fp64"],
+                "grad_rel_diff_vs_fp64_adjoint": float(np.linalg.norm(g - gr) /
+                                                       np.linalg.norm(gr))})
+    _emit(res)
     return 0
 
 
