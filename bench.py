@@ -101,8 +101,7 @@ def run_c3(args):
     log(f"factor done in {t_factor:.1f}s (elimination {t_elim:.1f}s), nnz(G)={nnz_g}, "
         f"fill={2 * nnz_g / A.nnz:.3f}")
     P = po.astype(np.int64)
-    Ap = A[P][:, P].tocsr()
-    Ap.sort_indices()
+    Ap = F.permute(A, po)
     del A
     b = np.random.default_rng(seeds["b"]).uniform(-1.0, 1.0, n)
     bp = b[P]
@@ -328,8 +327,7 @@ def run_c2(args):
     for order in ("mis", "static"):
         Gs, po, tf = F.factor(A, perm, blocks, order, 1)
         P = po.astype(np.int64)
-        Ap = A[P][:, P].tocsr()
-        Ap.sort_indices()
+        Ap = F.permute(A, po)
         r = _solve_pair(Ap, Gs, b[P], args, f"c2/{order}")
         r["fill"] = 2 * Gs[1].size / A.nnz
         if res is None:
@@ -353,9 +351,7 @@ def run_c4a(args):
     perm, blocks = F.nd_grid(s, s, s, depth)
     log("c4a: factor")
     Gs, po, _ = F.factor(A, perm, blocks, args.order, 1)
-    P = po.astype(np.int64)
-    Ap = A[P][:, P].tocsr()
-    Ap.sort_indices()
+    Ap = F.permute(A, po)
     del A
     k = args.nrhs
     B = np.random.default_rng(11).uniform(-1, 1, (n, k))

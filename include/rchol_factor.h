@@ -82,6 +82,11 @@ int rchol_grid_laplacian(int nx, int ny, int nz, const double *w, double shift, 
  * m = 3*nx*ny*nz - nx*ny - ny*nz - nx*nz edges, u[e] < v[e]. */
 int rchol_grid_edges(int nx, int ny, int nz, size_t *m, size_t **u, size_t **v);
 
+/* B = A(perm, perm) with sorted rows: the reference's reorder(A, P, B)
+ * (c++/util/util.cpp:16-56), threaded. */
+int rchol_permute_csr(size_t n, const size_t *rowPtr, const size_t *colIdx, const double *val,
+                      const size_t *perm, size_t **oRowPtr, size_t **oColIdx, double **oVal);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,6 +85,8 @@ FACTOR_SYMBOLS = {
                                             ctypes.c_double, size_tp, pu64p, pu64p, pf64p]),
     "rchol_grid_edges": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, size_tp,
                                         pu64p, pu64p]),
+    "rchol_permute_csr": (ctypes.c_int, [ctypes.c_size_t, u64p, u64p, f64p, u64p, pu64p, pu64p,
+                                         pf64p]),
 }
 
 _engine = None

@@ -43,6 +43,7 @@
 #include <vector>
 
 #include "../../../include/rchol_factor.h"
+#include "parallel.hpp"
 
 namespace {
 
@@ -341,26 +342,27 @@ int rchol_factor(size_t n, const size_t *rowPtr, const size_t *colIdx, const dou
     // (util.cpp:16-56 reorder + util.hpp:60-142 + rchol_lap.cpp:778-855).
     F.cols.resize(n + 1);
     F.diag.assign(n + 1, 0.0);
-    std::vector<Ent> row;
-    for (size_t i = 0; i < n; i++) {
-      const size_t o = p[i];
-      row.clear();
-      for (size_t k = rowPtr[o]; k < rowPtr[o + 1]; k++) {
-        if (colIdx[k] >= n) return fail(-1, "column index out of range");
-        row.push_back({transp[colIdx[k]], val[k]});
+    for (size_t k = 0; k < rowPtr[n]; k++)
+      if (colIdx[k] >= n) return fail(-1, "column index out of range");
+    rchol_host::parallel_for(n, [&](size_t i0, size_t i1) {
+      std::vector<Ent> row;
+      for (size_t i = i0; i < i1; i++) {
+        const size_t o = p[i];
+        row.clear();
+        for (size_t k = rowPtr[o]; k < rowPtr[o + 1]; k++) row.push_back({transp[colIdx[k]], val[k]});
+        if (perm) std::sort(row.begin(), row.end(), ByRow());
+        double sum = 0.0;
+        for (const Ent &x : row) sum += x.w;
+        Column &col = F.cols[i];
+        for (const Ent &x : row)
+          if (x.row >= i) col.e.push_back({x.row, x.w * -1});
+        if (std::abs(sum) > 1e-09) {
+          const double g = (-sum) * -1;
+          if (!(g < 0)) col.e.push_back({n, g});
+        }
+        col.split = col.e.size();
       }
-      if (perm) std::sort(row.begin(), row.end(), ByRow());
-      double sum = 0.0;
-      for (const Ent &x : row) sum += x.w;
-      Column &col = F.cols[i];
-      for (const Ent &x : row)
-        if (x.row >= i) col.e.push_back({x.row, x.w * -1});
-      if (std::abs(sum) > 1e-09) {
-        const double g = (-sum) * -1;
-        if (!(g < 0)) col.e.push_back({n, g});
-      }
-      col.split = col.e.size();
-    }
+    });
     F.pos.assign(n + 1, 0);
     F.level.reset(new std::atomic<int>[n + 1]);
     for (size_t i = 0; i <= n; i++) F.level[i].store(0, std::memory_order_relaxed);
@@ -372,48 +374,56 @@ int rchol_factor(size_t n, const size_t *rowPtr, const size_t *colIdx, const dou
     // coalesce (rchol_lap.cpp:358-387), rows mapped to final positions
     std::vector<size_t> col_at(n);
     for (size_t v = 0; v < n; v++) col_at[F.pos[v]] = v;
-    size_t nnz = 0;
-    for (size_t v = 0; v < n; v++)
-      for (const Ent &x : F.cols[v].e) nnz += (x.row != n);
+    std::vector<size_t> cnt(n + 1, 0);
+    rchol_host::parallel_for(n, [&](size_t c0, size_t c1) {
+      for (size_t c = c0; c < c1; c++) {
+        size_t k = 0;
+        for (const Ent &x : F.cols[col_at[c]].e) k += (x.row != n);
+        cnt[c + 1] = k;
+      }
+    });
+    for (size_t c = 0; c < n; c++) cnt[c + 1] += cnt[c];
+    const size_t nnz = cnt[n];
     out->n = n;
     out->colPtr = (size_t *)malloc((n + 1) * sizeof(size_t));
     out->rowIdx = (size_t *)malloc(std::max<size_t>(nnz, 1) * sizeof(size_t));
     out->val = (double *)malloc(std::max<size_t>(nnz, 1) * sizeof(double));
     out->perm = (size_t *)malloc(n * sizeof(size_t));
     if (!out->colPtr || !out->rowIdx || !out->val || !out->perm) return fail(-2, "out of memory");
-    size_t cnt = 0;
-    out->colPtr[0] = 0;
-    std::vector<Ent> tmp;
-    for (size_t c = 0; c < n; c++) {
-      const size_t v = col_at[c];
-      const std::vector<Ent> &e = F.cols[v].e;
-      const double sd = std::sqrt(F.diag[v]);
-      if (order == RCHOL_ORDER_STATIC) {
-        for (const Ent &x : e) {
-          if (x.row == n) continue;
-          double g = x.w;
-          if (x.row != v) g = g * -1;
-          out->rowIdx[cnt] = x.row;
-          out->val[cnt] = g * sd;
-          cnt++;
-        }
-      } else {
-        tmp.clear();
-        for (const Ent &x : e) {
-          if (x.row == n) continue;
-          double g = x.w;
-          if (x.row != v) g = g * -1;
-          tmp.push_back({F.pos[x.row], g * sd});
-        }
-        std::sort(tmp.begin(), tmp.end(), ByRow());
-        for (const Ent &x : tmp) {
-          out->rowIdx[cnt] = x.row;
-          out->val[cnt] = x.w;
-          cnt++;
+    std::copy(cnt.begin(), cnt.end(), out->colPtr);
+    rchol_host::parallel_for(n, [&](size_t c0, size_t c1) {
+      std::vector<Ent> tmp;
+      for (size_t c = c0; c < c1; c++) {
+        const size_t v = col_at[c];
+        const std::vector<Ent> &e = F.cols[v].e;
+        const double sd = std::sqrt(F.diag[v]);
+        size_t at = cnt[c];
+        if (order == RCHOL_ORDER_STATIC) {
+          for (const Ent &x : e) {
+            if (x.row == n) continue;
+            double g = x.w;
+            if (x.row != v) g = g * -1;
+            out->rowIdx[at] = x.row;
+            out->val[at] = g * sd;
+            at++;
+          }
+        } else {
+          tmp.clear();
+          for (const Ent &x : e) {
+            if (x.row == n) continue;
+            double g = x.w;
+            if (x.row != v) g = g * -1;
+            tmp.push_back({F.pos[x.row], g * sd});
+          }
+          std::sort(tmp.begin(), tmp.end(), ByRow());
+          for (const Ent &x : tmp) {
+            out->rowIdx[at] = x.row;
+            out->val[at] = x.w;
+            at++;
+          }
         }
       }
-      out->colPtr[c + 1] = cnt;
-    }
+    });
     for (size_t v = 0; v < n; v++) out->perm[F.pos[v]] = p[v];
     out->seconds = std::chrono::duration<double>(t1 - t0).count();
   } catch (const std::exception &ex) {

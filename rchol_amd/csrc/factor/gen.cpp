@@ -1,5 +1,6 @@
 // Problem generators (host): the matrices of the benchmark and test
 // configurations, as 64-bit CSR with sorted rows.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -7,6 +8,7 @@
 #include <vector>
 
 #include "../../../include/rchol_factor.h"
+#include "parallel.hpp"
 
 namespace {
 
@@ -145,6 +147,47 @@ int rchol_grid_laplacian(int nx, int ny, int nz, const double *wt, double shift,
       }
   }
   return assemble7(nx, ny, nz, w, diag, n, rowPtr, colIdx, val);
+}
+
+// util.cpp:16-56: B = A(P, P), row i of B is row P[i] of A with columns
+// renumbered by the inverse permutation and sorted (threaded).
+int rchol_permute_csr(size_t n, const size_t *rowPtr, const size_t *colIdx, const double *val,
+                      const size_t *perm, size_t **oRowPtr, size_t **oColIdx, double **oVal) {
+  if (!rowPtr || !colIdx || !val || !perm || !oRowPtr || !oColIdx || !oVal) return -1;
+  std::vector<size_t> inv(n, (size_t)-1);
+  for (size_t i = 0; i < n; i++) {
+    if (perm[i] >= n || inv[perm[i]] != (size_t)-1) return -1;
+    inv[perm[i]] = i;
+  }
+  const size_t nnz = rowPtr[n];
+  size_t *rp = (size_t *)malloc((n + 1) * sizeof(size_t));
+  size_t *ci = (size_t *)malloc(std::max<size_t>(nnz, 1) * sizeof(size_t));
+  double *v = (double *)malloc(std::max<size_t>(nnz, 1) * sizeof(double));
+  if (!rp || !ci || !v) return -2;
+  rp[0] = 0;
+  for (size_t i = 0; i < n; i++) rp[i + 1] = rp[i] + (rowPtr[perm[i] + 1] - rowPtr[perm[i]]);
+  rchol_host::parallel_for(n, [&](size_t i0, size_t i1) {
+    std::vector<std::pair<size_t, double>> row;
+    for (size_t i = i0; i < i1; i++) {
+      const size_t o = perm[i];
+      row.clear();
+      for (size_t k = rowPtr[o]; k < rowPtr[o + 1]; k++) row.push_back({inv[colIdx[k]], val[k]});
+      std::sort(row.begin(), row.end(),
+                [](const std::pair<size_t, double> &a, const std::pair<size_t, double> &b) {
+                  return a.first < b.first;
+                });
+      size_t at = rp[i];
+      for (auto &e : row) {
+        ci[at] = e.first;
+        v[at] = e.second;
+        at++;
+      }
+    }
+  });
+  *oRowPtr = rp;
+  *oColIdx = ci;
+  *oVal = v;
+  return 0;
 }
 
 }  // extern "C"

@@ -79,6 +79,22 @@ def grid_laplacian(nx, ny, nz, w, shift):
     return _take_csr(n, rp, ci, v)
 
 
+def permute(A, perm):
+    """A(perm, perm) with sorted rows (util.cpp:16-56 reorder), threaded."""
+    A = sp.csr_matrix(A)
+    n = A.shape[0]
+    rp = np.ascontiguousarray(A.indptr, dtype=np.uint64)
+    ci = np.ascontiguousarray(A.indices, dtype=np.uint64)
+    v = np.ascontiguousarray(A.data, dtype=np.float64)
+    p = np.ascontiguousarray(perm, dtype=np.uint64)
+    orp, oci, ov = _lib.u64p(), _lib.u64p(), _lib.f64p()
+    _chk(_lib.factor().rchol_permute_csr(n, rp.ctypes.data_as(_lib.u64p),
+                                         ci.ctypes.data_as(_lib.u64p), v.ctypes.data_as(_lib.f64p),
+                                         p.ctypes.data_as(_lib.u64p), ctypes.byref(orp),
+                                         ctypes.byref(oci), ctypes.byref(ov)))
+    return _take_csr(ctypes.c_size_t(n), orp, oci, ov)
+
+
 def nd_grid(nx, ny, nz, depth):
     """Geometric nested dissection -> (perm, block sizes in post-order)."""
     n = int(nx) * int(ny) * int(nz)
