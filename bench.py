@@ -177,9 +177,12 @@ def run_c3(args):
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
+    copy = s32.copy_probe(1 << 30)
+    log(f"plain copy: {copy['gbs']:.0f} GB/s")
     roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["gbs"], "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": dk["gbs"] / HBM_PEAK_GBS, "traffic": traffic,
-                "algorithmic_bytes": dk["bytes"], "avg_ms": dk["avg_ms"]}
+                "algorithmic_bytes": dk["bytes"], "avg_ms": dk["avg_ms"],
+                "copy_gbs": copy["gbs"]}
 
     cpu = None
     if not args.no_cpu:

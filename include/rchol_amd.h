@@ -171,6 +171,11 @@ int rchol_amd_edge_grad_device(rchol_amd_ctx *ctx, size_t m, const long long *d_
 int rchol_amd_autotune(rchol_amd_ctx *ctx, int reps, int *trsv_lanes, int *spmv_lanes,
                        double *trsv_ms, double *spmv_ms);
 
+/* Streaming device copy of `bytes` (median of reps launches): the achievable
+ * HBM bandwidth (read + write bytes / time) and the known-size kernel used to
+ * scale profiler byte counters.  No reference counterpart. */
+int rchol_amd_copy_probe(rchol_amd_ctx *ctx, size_t bytes, int reps, double *gbs, double *ms);
+
 int rchol_amd_set_option(rchol_amd_ctx *ctx, int key, double value);
 int rchol_amd_get_stats(rchol_amd_ctx *ctx, rchol_amd_stats *out);
 

@@ -52,6 +52,8 @@ ENGINE_SYMBOLS = {
     "rchol_amd_set_segments": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, u64p]),
     "rchol_amd_pcg_segments": (ctypes.c_int, [ctypes.c_void_p, f64p, ctypes.c_double, ctypes.c_int,
                                               f64p, ctypes.c_int, f64p, i32p]),
+    "rchol_amd_copy_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, f64p,
+                                            f64p]),
     "rchol_amd_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, i32p, i32p, f64p, f64p]),
     "rchol_amd_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]),
     "rchol_amd_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

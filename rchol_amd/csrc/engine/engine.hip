@@ -1610,6 +1610,35 @@ int rchol_amd_autotune(rchol_amd_ctx *c, int reps, int *trsv_lanes, int *spmv_la
   return RCHOL_AMD_OK;
 }
 
+
+int rchol_amd_copy_probe(rchol_amd_ctx *c, size_t bytes, int reps, double *gbs, double *ms) {
+  if (!c || !gbs) return fail(RCHOL_AMD_EINVAL, "null argument");
+  if (bytes < 4096) return fail(RCHOL_AMD_EINVAL, "copy probe needs at least 4 KiB");
+  if (reps < 1) reps = 5;
+  HIP_TRY(hipSetDevice(c->device));
+  const size_t n16 = bytes / 16;
+  DevBuf<double2> a, b;
+  HIP_TRY(a.alloc(n16));
+  HIP_TRY(b.alloc(n16));
+  HIP_TRY(hipMemsetAsync(a.p, 0, n16 * 16, c->stream));
+  const int g = c->ncu * 8;
+  std::vector<float> t;
+  for (int r = 0; r <= reps; r++) {
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_copy, dim3(g), dim3(kBlock), 0, c->stream, n16, (const double2 *)a.p, b.p);
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float m = 0.f;
+    HIP_TRY(hipEventElapsedTime(&m, c->ev0, c->ev1));
+    if (r) t.push_back(m);
+  }
+  std::sort(t.begin(), t.end());
+  const double med = t[t.size() / 2];
+  *gbs = 2.0 * (double)(n16 * 16) / (med * 1e-3) / 1e9;
+  if (ms) *ms = med;
+  return RCHOL_AMD_OK;
+}
+
 int rchol_amd_pcg_csr(size_t n, const size_t *aRowPtr, const size_t *aColIdx, const double *aVal,
                       const double *b, double tol, int maxit, const size_t *gColPtr,
                       const size_t *gRowIdx, const double *gVal, double *x, int x_in_is_initial,

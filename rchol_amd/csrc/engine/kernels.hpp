@@ -106,6 +106,15 @@ __global__ void k_permute(int n, int K, const int *__restrict__ perm, const doub
   }
 }
 
+// Plain streaming copy (16-byte vectors): the reference kernel for the
+// achievable HBM bandwidth and for scaling the profiler's byte counters.
+__global__ __launch_bounds__(256) void k_copy(size_t n16, const double2 *__restrict__ a,
+                                              double2 *__restrict__ b) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
 // Gradient of a loss through x = A(w)^-1 b with respect to the weight w_e of
 // edge (u_e, v_e) of A = sum_e w_e (e_u - e_v)(e_u - e_v)^T + D:
 // dL/dw_e = -(lam_u - lam_v)(x_u - x_v), lam = A^-1 dL/dx.

@@ -136,6 +136,13 @@ class Solver:
         if self.n is None:
             self.n = n
 
+    def copy_probe(self, nbytes=1 << 30, reps=5):
+        """Achievable HBM GB/s of a plain streaming copy (read + write)."""
+        g, m = ctypes.c_double(), ctypes.c_double()
+        _lib.check(self._lib.rchol_amd_copy_probe(self._h, int(nbytes), int(reps), ctypes.byref(g),
+                                                  ctypes.byref(m)))
+        return {"gbs": g.value, "ms": m.value, "bytes": int(nbytes)}
+
     def autotune(self, reps=3):
         """Pick lanes-per-row of the sweeps and the SpMV by timing them."""
         tl, sl = ctypes.c_int(), ctypes.c_int()
