@@ -43,6 +43,8 @@ typedef struct rchol_amd_ctx rchol_amd_ctx;
 #define RCHOL_AMD_OPT_SPMV_LANES 3    /* lanes per row in SpMV (4, 8, 16 or 32) */
 #define RCHOL_AMD_OPT_TRSV_LANES 4    /* lanes per row in the triangular sweeps */
 #define RCHOL_AMD_OPT_PROFILE 5       /* 1: time every kernel of the loop with HIP events */
+#define RCHOL_AMD_OPT_SERIAL_WIDTH 6  /* leading/trailing sweep levels of at most this many
+                                         rows run in one workgroup (0 = off) */
 
 /* Counters filled by rchol_amd_get_stats. */
 typedef struct rchol_amd_stats {

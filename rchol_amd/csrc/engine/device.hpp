@@ -20,6 +20,7 @@ struct DevScalars {
   double tol;
   double relres;
   double qq;        // ||A x - b||^2 (final true residual)
+  double rz_acc;    // r.z accumulated by the phases of a split backward sweep
   int k;            // completed iterations
   int maxit;
   int done;         // 1 once the loop test of pcg.cpp:82 fails

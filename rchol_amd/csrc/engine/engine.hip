@@ -79,13 +79,18 @@ struct Sweep {
   int nslots = 0, ntasks = 0, nlevels = 0;
   size_t nnz_off = 0;
   DevBuf<int> task_ptr, slot_row, rp, ci;
+  // split schedule: levels [0, head_end) and [tail_begin, nlevels) run in one
+  // workgroup (k_sptrsv_serial), the middle in the persistent kernel
+  std::vector<int> level_ptr_h;
+  DevBuf<int> level_ptr, mid_task_ptr;
+  int head_end = 0, tail_begin = 0, mid_ntasks = 0;
   DevBuf<double> cv64, diag64;
   DevBuf<float> cv32, diag32;
 };
 
 // Host-side schedule of one sweep before upload.
 struct HostSweep {
-  std::vector<int> task_ptr, slot_row, rp, ci;
+  std::vector<int> task_ptr, slot_row, rp, ci, level_ptr;
   std::vector<double> cv, diag;
   int nlevels = 0;
 };
@@ -110,6 +115,28 @@ void build_slots(const std::vector<int> &lev, int nlev, HostSweep &hs, std::vect
   }
   hs.task_ptr.push_back(n);
   hs.nlevels = nlev;
+  hs.level_ptr.assign(cnt.begin(), cnt.end());
+}
+
+// Cut a sweep's levels into a narrow head, a middle and a narrow tail
+// (levels of at most `width` rows run in one workgroup; width 0 = no split)
+// and upload the middle's task list.
+hipError_t split_levels(Sweep &sw, int width) {
+  const std::vector<int> &lp = sw.level_ptr_h;
+  const int L = sw.nlevels;
+  int h = 0, t = L;
+  if (width > 0) {
+    while (h < L && lp[h + 1] - lp[h] <= width) h++;
+    while (t > h && lp[t] - lp[t - 1] <= width) t--;
+  }
+  std::vector<int> tasks;
+  for (int l = h; l < t; l++)
+    for (int s = lp[l]; s < lp[l + 1]; s += kTaskRows) tasks.push_back(s);
+  tasks.push_back(t > h ? lp[t] : lp[h]);
+  sw.head_end = h;
+  sw.tail_begin = t;
+  sw.mid_ntasks = (int)tasks.size() - 1;
+  return sw.mid_task_ptr.upload(tasks.data(), tasks.size());
 }
 
 }  // namespace
@@ -149,6 +176,7 @@ struct rchol_amd_ctx {
   int trsv_lanes = 8;
   int profile = 0;
   int profile_no_graph = 0;
+  int serial_width = 0;  // narrow-level threshold of the split sweeps (0 = off)
   std::vector<hipEvent_t> pev;  // 5 per captured iteration when profiling
   // captured iteration graph
   hipGraph_t graph = nullptr;
@@ -251,6 +279,28 @@ namespace {
 
 // Launch the forward + backward sweeps.  pcg_mode: fused sentinel/dot
 // epilogues for the PCG loop; otherwise a standalone apply z = M^-1 r.
+// One sweep in up to three launches: serial head, persistent middle, serial
+// tail (the split is empty unless RCHOL_AMD_OPT_SERIAL_WIDTH is set).
+// FUSE 1: sentinel into aux; 2: r.z -> beta; 0: plain.
+template <typename VT, typename XT, typename BT, int L, int FUSE>
+void launch_one_sweep(rchol_amd_ctx *c, const Sweep &sw, const VT *cv, const VT *dg,
+                      const BT *rhs, XT *x, XT *aux, const double *dotv, const int *stop) {
+  const bool head = sw.head_end > 0, tail = sw.tail_begin < sw.nlevels;
+  if (head)
+    hipLaunchKernelGGL((k_sptrsv_serial<VT, XT, BT, L, FUSE>), dim3(1), dim3(1024), 0, c->stream,
+                       0, sw.head_end, sw.level_ptr.p, sw.slot_row.p, sw.rp.p, sw.ci.p, cv, dg,
+                       rhs, x, aux, dotv, c->sc, stop, 0);
+  if (sw.mid_ntasks > 0 || (FUSE == 2 && !tail))
+    hipLaunchKernelGGL((k_sptrsv<VT, XT, BT, L, FUSE>), dim3(c->trsv_grid), dim3(kBlock), 0,
+                       c->stream, sw.mid_ntasks, sw.mid_task_ptr.p, sw.slot_row.p, sw.rp.p,
+                       sw.ci.p, cv, dg, rhs, x, aux, dotv, c->partials.p, c->sc, stop,
+                       tail ? 1 : 0);
+  if (tail)
+    hipLaunchKernelGGL((k_sptrsv_serial<VT, XT, BT, L, FUSE>), dim3(1), dim3(1024), 0, c->stream,
+                       sw.tail_begin, sw.nlevels, sw.level_ptr.p, sw.slot_row.p, sw.rp.p, sw.ci.p,
+                       cv, dg, rhs, x, aux, dotv, c->sc, stop, 1);
+}
+
 template <typename VT, typename XT, int L>
 int launch_sweeps_t(rchol_amd_ctx *c, const double *rhs, XT *y, XT *z, const int *stop,
                     bool with_dot, const double *dotv, hipEvent_t *ev) {
@@ -260,21 +310,14 @@ int launch_sweeps_t(rchol_amd_ctx *c, const double *rhs, XT *y, XT *z, const int
   } else {
     fcv = c->fwd.cv32.p; fdg = c->fwd.diag32.p; bcv = c->bwd.cv32.p; bdg = c->bwd.diag32.p;
   }
-  hipLaunchKernelGGL((k_sptrsv<VT, XT, double, L, 1>), dim3(c->trsv_grid), dim3(kBlock), 0,
-                     c->stream, c->fwd.ntasks, c->fwd.task_ptr.p, c->fwd.slot_row.p, c->fwd.rp.p,
-                     c->fwd.ci.p, fcv, fdg, rhs, y, z, nullptr, nullptr, c->sc, stop);
+  launch_one_sweep<VT, XT, double, L, 1>(c, c->fwd, fcv, fdg, rhs, y, z, nullptr, stop);
   if (ev) (void)hipEventRecord(ev[1], c->stream);
-  if (with_dot) {
-    hipLaunchKernelGGL((k_sptrsv<VT, XT, XT, L, 2>), dim3(c->trsv_grid), dim3(kBlock), 0,
-                       c->stream, c->bwd.ntasks, c->bwd.task_ptr.p, c->bwd.slot_row.p,
-                       c->bwd.rp.p, c->bwd.ci.p, bcv, bdg, (const XT *)y, z, (XT *)nullptr, dotv,
-                       c->partials.p, c->sc, stop);
-  } else {
-    hipLaunchKernelGGL((k_sptrsv<VT, XT, XT, L, 0>), dim3(c->trsv_grid), dim3(kBlock), 0,
-                       c->stream, c->bwd.ntasks, c->bwd.task_ptr.p, c->bwd.slot_row.p,
-                       c->bwd.rp.p, c->bwd.ci.p, bcv, bdg, (const XT *)y, z, (XT *)nullptr,
-                       nullptr, nullptr, c->sc, stop);
-  }
+  if (with_dot)
+    launch_one_sweep<VT, XT, XT, L, 2>(c, c->bwd, bcv, bdg, (const XT *)y, z, (XT *)nullptr, dotv,
+                                       stop);
+  else
+    launch_one_sweep<VT, XT, XT, L, 0>(c, c->bwd, bcv, bdg, (const XT *)y, z, (XT *)nullptr,
+                                       nullptr, stop);
   if (ev) (void)hipEventRecord(ev[2], c->stream);
   return RCHOL_AMD_OK;
 }
@@ -1187,6 +1230,9 @@ int rchol_amd_upload_G(rchol_amd_ctx *c, size_t n, const size_t *colPtr, const s
     sw.nlevels = h.nlevels;
     sw.nnz_off = noff;
     HIP_TRY(sw.task_ptr.upload(h.task_ptr.data(), h.task_ptr.size()));
+    sw.level_ptr_h = h.level_ptr;
+    HIP_TRY(sw.level_ptr.upload(h.level_ptr.data(), h.level_ptr.size()));
+    HIP_TRY(split_levels(sw, c->serial_width));
     HIP_TRY(sw.slot_row.upload(h.slot_row.data(), n));
     HIP_TRY(sw.rp.upload(h.rp.data(), n + 1));
     HIP_TRY(sw.ci.upload(h.ci.data(), noff));
@@ -1246,6 +1292,14 @@ int rchol_amd_set_option(rchol_amd_ctx *c, int key, double value) {
       break;
     case RCHOL_AMD_OPT_PROFILE:
       c->profile = v ? 1 : 0;
+      break;
+    case RCHOL_AMD_OPT_SERIAL_WIDTH:
+      if (v < 0 || v > 1 << 20) return fail(RCHOL_AMD_EINVAL, "serial width %d out of range", v);
+      c->serial_width = v;
+      if (c->hasG) {
+        HIP_TRY(split_levels(c->fwd, v));
+        HIP_TRY(split_levels(c->bwd, v));
+      }
       break;
     default:
       return fail(RCHOL_AMD_EINVAL, "unknown option key %d", key);

@@ -35,7 +35,7 @@ __global__ __launch_bounds__(256) void k_sptrsv(
     const int *__restrict__ rp, const int *__restrict__ ci, const VT *__restrict__ cv,
     const VT *__restrict__ diag, const BT *__restrict__ rhs, XT *x, XT *__restrict__ aux,
     const double *__restrict__ dotv, double *__restrict__ partials, DevScalars *sc,
-    const int *stop) {
+    const int *stop, int defer) {
   if (*stop) return;
   constexpr int R = 64 / L;
   const int lane = threadIdx.x & 63;
@@ -67,10 +67,71 @@ __global__ __launch_bounds__(256) void k_sptrsv(
   if (FUSE == 2) {
     double v[1] = {dacc}, tot[1];
     if (grid_reduce<1>(v, partials, &sc->ctr[CTR_BWD], tot)) {
-      const double rz = tot[0];
-      sc->rz = rz;
-      sc->beta = (sc->k > 0) ? rz / sc->rz_old : 0.0;
-      sc->rz_old = rz;
+      if (defer) {
+        sc->rz_acc += tot[0];  // a serial tail phase finalises
+      } else {
+        const double rz = sc->rz_acc + tot[0];
+        sc->rz_acc = 0.0;
+        sc->rz = rz;
+        sc->beta = (sc->k > 0) ? rz / sc->rz_old : 0.0;
+        sc->rz_old = rz;
+      }
+    }
+  }
+}
+
+// Narrow levels in one workgroup: levels [lev0, lev1) of a sweep, one after
+// the other, separated by workgroup barriers.  It runs as its own launch
+// before (head) or after (tail) the persistent kernel, so every dependency
+// outside the range is complete at launch and every one inside it was
+// written by this workgroup before a barrier: plain loads, no polling, no
+// cross-CU hop per level.
+// FUSE == 1: sentinel into aux.  FUSE == 2: r.z partial of these rows;
+// fin == 0 adds it to sc->rz_acc (a later phase finalises), fin == 1 adds
+// sc->rz_acc and finalises rz / beta (pcg.cpp:93-95).
+template <typename VT, typename XT, typename BT, int L, int FUSE>
+__global__ __launch_bounds__(1024) void k_sptrsv_serial(
+    int lev0, int lev1, const int *__restrict__ level_ptr, const int *__restrict__ slot_row,
+    const int *__restrict__ rp, const int *__restrict__ ci, const VT *__restrict__ cv,
+    const VT *__restrict__ diag, const BT *__restrict__ rhs, XT *x, XT *__restrict__ aux,
+    const double *__restrict__ dotv, DevScalars *sc, const int *stop, int fin) {
+  if (*stop) return;
+  const int G = blockDim.x / L;
+  const int g = threadIdx.x / L, sub = threadIdx.x % L;
+  double dacc = 0.0;
+  for (int l = lev0; l < lev1; l++) {
+    const int s0 = level_ptr[l], s1 = level_ptr[l + 1];
+    for (int sb = s0; sb < s1; sb += G) {
+      const int s = sb + g;
+      XT acc = 0;
+      int row = 0;
+      if (s < s1) {
+        row = slot_row[s];
+        for (int e = rp[s] + sub; e < rp[s + 1]; e += L) acc += (XT)cv[e] * x[ci[e]];
+      }
+      acc = group_sum<L>(acc);
+      if (s < s1 && sub == 0) {
+        const XT v = ((XT)rhs[row] - acc) / (XT)diag[s];
+        x[row] = v;
+        if (FUSE == 1) aux[row] = Sent<XT>::value();
+        if (FUSE == 2) dacc += dotv[row] * (double)v;
+      }
+    }
+    __syncthreads();
+  }
+  if (FUSE == 2) {
+    double v[1] = {dacc};
+    block_sum<1>(v);
+    if (threadIdx.x == 0) {
+      if (fin) {
+        const double rz = sc->rz_acc + v[0];
+        sc->rz_acc = 0.0;
+        sc->rz = rz;
+        sc->beta = (sc->k > 0) ? rz / sc->rz_old : 0.0;
+        sc->rz_old = rz;
+      } else {
+        sc->rz_acc += v[0];
+      }
     }
   }
 }
@@ -153,6 +214,7 @@ __global__ __launch_bounds__(256) void k_init(int n, const double *__restrict__ 
     sc->rr = tot[0];
     sc->k = 0;
     sc->err = 0;
+    sc->rz_acc = 0.0;
     sc->beta = 0.0;
     sc->done = !(sqrt(tot[0]) > sqrt(tot[0]) * sc->tol && 0 < sc->maxit);
   }

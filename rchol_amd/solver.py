@@ -19,6 +19,7 @@ OPT_RESID_REPLACE = 2
 OPT_SPMV_LANES = 3
 OPT_TRSV_LANES = 4
 OPT_PROFILE = 5
+OPT_SERIAL_WIDTH = 6
 
 
 class Stats(ctypes.Structure):

@@ -258,3 +258,23 @@ def test_autotune_keeps_results():
         x, relres, itr = s.pcg(d["b"], 1e-10, 5000)
     assert rel(z, d["z"]) <= PRECOND_RTOL
     assert abs(itr - int(d["itr"])) <= 1
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+@pytest.mark.parametrize("width", [16, 1 << 20])
+@pytest.mark.parametrize("precision", [rchol_amd.FP64, rchol_amd.FP32])
+def test_split_sweeps_match_oracle(name, width, precision):
+    A, Gs, d = load(name)
+    print(f"case split {name} width={width} prec={precision}", flush=True)
+    with rchol_amd.Solver(A, Gs, precision=precision) as s:
+        s.set_option(rchol_amd.solver.OPT_SERIAL_WIDTH, width)
+        z = s.precond(d["b"])
+        x, relres, itr = s.pcg(d["b"], 1e-10, 5000)
+    true = np.linalg.norm(A @ x - d["b"]) / np.linalg.norm(d["b"])
+    if precision == rchol_amd.FP64:
+        assert rel(z, d["z"]) <= PRECOND_RTOL
+        assert abs(itr - int(d["itr"])) <= 1
+    else:
+        assert rel(z, d["z"]) <= 1e-5
+        assert itr <= int(d["itr"]) + 5
+    assert true <= 1e-9 and relres == pytest.approx(true, rel=1e-6)
