@@ -67,6 +67,16 @@ typedef struct rchol_amd_stats {
   int kernel_iters;
 } rchol_amd_stats;
 
+/* Level schedule of the two triangular sweeps (host analysis only). */
+typedef struct rchol_amd_schedule {
+  int nlevels[2];    /* [0] forward G y = r, [1] backward G^T z = y */
+  int head_end[2];   /* levels [0, head_end) run in one workgroup */
+  int tail_begin[2]; /* levels [tail_begin, nlevels) run in one workgroup */
+  int mid_tasks[2];  /* tasks of the persistent kernel in between */
+  int *level_ptr[2]; /* nlevels+1 slot offsets per level (malloc'ed) */
+  int *slot_row[2];  /* n: row held by each slot, in level order (malloc'ed) */
+} rchol_amd_schedule;
+
 /* Thread-local text of the last error on the calling thread. */
 const char *rchol_amd_last_error(void);
 
@@ -180,6 +190,13 @@ int rchol_amd_copy_probe(rchol_amd_ctx *ctx, size_t bytes, int reps, double *gbs
 
 int rchol_amd_set_option(rchol_amd_ctx *ctx, int key, double value);
 int rchol_amd_get_stats(rchol_amd_ctx *ctx, rchol_amd_stats *out);
+
+/* Host-only analysis of a stored factor (no device needed): the validation
+ * and level schedule that rchol_amd_upload_G builds, for inspection and
+ * tests.  Free with rchol_amd_schedule_free.  No reference counterpart. */
+int rchol_amd_analyze_G(size_t n, const size_t *colPtr, const size_t *rowIdx, const double *val,
+                        int serial_width, rchol_amd_schedule *out);
+void rchol_amd_schedule_free(rchol_amd_schedule *s);
 
 /* Number of visible HIP devices (0 when none); never fails. */
 int rchol_amd_device_count(void);

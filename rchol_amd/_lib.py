@@ -17,8 +17,17 @@ u64p = ctypes.POINTER(ctypes.c_uint64)
 f64p = ctypes.POINTER(ctypes.c_double)
 i32p = ctypes.POINTER(ctypes.c_int)
 
+class Schedule(ctypes.Structure):
+    _fields_ = [("nlevels", ctypes.c_int * 2), ("head_end", ctypes.c_int * 2),
+                ("tail_begin", ctypes.c_int * 2), ("mid_tasks", ctypes.c_int * 2),
+                ("level_ptr", i32p * 2), ("slot_row", i32p * 2)]
+
+
 # Every C-ABI symbol of include/rchol_amd.h with its signature.
 ENGINE_SYMBOLS = {
+    "rchol_amd_analyze_G": (ctypes.c_int, [ctypes.c_size_t, u64p, u64p, f64p, ctypes.c_int,
+                                           ctypes.POINTER(Schedule)]),
+    "rchol_amd_schedule_free": (None, [ctypes.POINTER(Schedule)]),
     "rchol_amd_last_error": (ctypes.c_char_p, []),
     "rchol_amd_device_count": (ctypes.c_int, []),
     "rchol_amd_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),

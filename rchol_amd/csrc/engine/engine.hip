@@ -121,14 +121,19 @@ void build_slots(const std::vector<int> &lev, int nlev, HostSweep &hs, std::vect
 // Cut a sweep's levels into a narrow head, a middle and a narrow tail
 // (levels of at most `width` rows run in one workgroup; width 0 = no split)
 // and upload the middle's task list.
-hipError_t split_levels(Sweep &sw, int width) {
-  const std::vector<int> &lp = sw.level_ptr_h;
-  const int L = sw.nlevels;
-  int h = 0, t = L;
+void compute_split(const std::vector<int> &lp, int L, int width, int &h, int &t) {
+  h = 0;
+  t = L;
   if (width > 0) {
     while (h < L && lp[h + 1] - lp[h] <= width) h++;
     while (t > h && lp[t] - lp[t - 1] <= width) t--;
   }
+}
+
+hipError_t split_levels(Sweep &sw, int width) {
+  const std::vector<int> &lp = sw.level_ptr_h;
+  int h, t;
+  compute_split(lp, sw.nlevels, width, h, t);
   std::vector<int> tasks;
   for (int l = h; l < t; l++)
     for (int s = lp[l]; s < lp[l + 1]; s += kTaskRows) tasks.push_back(s);
@@ -1121,15 +1126,14 @@ int rchol_amd_upload_A(rchol_amd_ctx *c, size_t n, const size_t *rowPtr, const s
   return RCHOL_AMD_OK;
 }
 
-int rchol_amd_upload_G(rchol_amd_ctx *c, size_t n, const size_t *colPtr, const size_t *rowIdx,
-                       const double *val, int precision) {
-  if (!c || !colPtr || !rowIdx || !val) return fail(RCHOL_AMD_EINVAL, "null argument");
-  if (precision != RCHOL_AMD_FP64 && precision != RCHOL_AMD_FP32)
-    return fail(RCHOL_AMD_EINVAL, "unknown precision %d", precision);
-  if (n == 0 || n > kIdxMax) return fail(RCHOL_AMD_ERANGE, "n=%zu outside [1, 2^31)", n);
+// Host analysis of the stored factor: validation (lower triangle, one
+// nonzero diagonal per column), dependency levels of both sweeps and their
+// slot-ordered CSR.  Forward: row i of G waits for every column j < i with
+// G(i,j) != 0.  Backward (G^T z = y): row i waits for every j > i with
+// G(j,i) != 0, i.e. the stored column i.
+static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, const double *val,
+                          HostSweep &hf, HostSweep &hb, int &nlf, int &nlb) {
   const size_t nnz = colPtr[n];
-  if (colPtr[0] != 0) return fail(RCHOL_AMD_EINVAL, "colPtr[0] must be 0");
-  if (nnz > kIdxMax) return fail(RCHOL_AMD_ERANGE, "nnz(G)=%zu exceeds the int32 index range", nnz);
   // Validate lower-triangular columns and find the diagonal of each.
   std::vector<double> diag(n, 0.0);
   std::vector<int> offf(n, 0);  // off-diagonals per row of G (forward sweep)
@@ -1151,11 +1155,9 @@ int rchol_amd_upload_G(rchol_amd_ctx *c, size_t n, const size_t *colPtr, const s
     if (!seen || diag[j] == 0.0)
       return fail(RCHOL_AMD_EPIVOT, "G(%zu,%zu) is zero or missing", j, j);
   }
-  // Dependency levels.  Forward: row i of G waits for every column j < i
-  // with G(i,j) != 0.  Backward (G^T z = y): row i of G^T waits for every
-  // i < j with G(j,i) != 0, i.e. the stored column i.
   std::vector<int> levf(n, 0), levb(n, 0);
-  int nlf = 0, nlb = 0;
+  nlf = 0;
+  nlb = 0;
   for (size_t j = 0; j < n; j++) {
     const int lj = levf[j] + 1;
     nlf = std::max(nlf, levf[j] + 1);
@@ -1173,7 +1175,6 @@ int rchol_amd_upload_G(rchol_amd_ctx *c, size_t n, const size_t *colPtr, const s
     levb[jj] = m;
     nlb = std::max(nlb, m + 1);
   }
-  HostSweep hf, hb;
   std::vector<int> slot_of;
   // forward sweep: CSR of lower G in slot order
   build_slots(levf, nlf, hf, slot_of);
@@ -1220,6 +1221,23 @@ int rchol_amd_upload_G(rchol_amd_ctx *c, size_t n, const size_t *colPtr, const s
     hb.diag[s] = diag[row];
   }
 
+  return RCHOL_AMD_OK;
+}
+
+int rchol_amd_upload_G(rchol_amd_ctx *c, size_t n, const size_t *colPtr, const size_t *rowIdx,
+                       const double *val, int precision) {
+  if (!c || !colPtr || !rowIdx || !val) return fail(RCHOL_AMD_EINVAL, "null argument");
+  if (precision != RCHOL_AMD_FP64 && precision != RCHOL_AMD_FP32)
+    return fail(RCHOL_AMD_EINVAL, "unknown precision %d", precision);
+  if (n == 0 || n > kIdxMax) return fail(RCHOL_AMD_ERANGE, "n=%zu outside [1, 2^31)", n);
+  const size_t nnz = colPtr[n];
+  if (colPtr[0] != 0) return fail(RCHOL_AMD_EINVAL, "colPtr[0] must be 0");
+  if (nnz > kIdxMax) return fail(RCHOL_AMD_ERANGE, "nnz(G)=%zu exceeds the int32 index range", nnz);
+  HostSweep hf, hb;
+  int nlf = 0, nlb = 0;
+  int rc0 = analyze_factor(n, colPtr, rowIdx, val, hf, hb, nlf, nlb);
+  if (rc0) return rc0;
+  const size_t noff = nnz - n;
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->drop_graph();
@@ -1691,6 +1709,46 @@ int rchol_amd_copy_probe(rchol_amd_ctx *c, size_t bytes, int reps, double *gbs, 
   *gbs = 2.0 * (double)(n16 * 16) / (med * 1e-3) / 1e9;
   if (ms) *ms = med;
   return RCHOL_AMD_OK;
+}
+
+
+int rchol_amd_analyze_G(size_t n, const size_t *colPtr, const size_t *rowIdx, const double *val,
+                        int serial_width, rchol_amd_schedule *out) {
+  if (!colPtr || !rowIdx || !val || !out) return fail(RCHOL_AMD_EINVAL, "null argument");
+  if (n == 0 || n > kIdxMax) return fail(RCHOL_AMD_ERANGE, "n=%zu outside [1, 2^31)", n);
+  if (colPtr[0] != 0) return fail(RCHOL_AMD_EINVAL, "colPtr[0] must be 0");
+  if (colPtr[n] > kIdxMax) return fail(RCHOL_AMD_ERANGE, "nnz(G) exceeds the int32 index range");
+  HostSweep h[2];
+  int nl[2];
+  int rc = analyze_factor(n, colPtr, rowIdx, val, h[0], h[1], nl[0], nl[1]);
+  if (rc) return rc;
+  memset(out, 0, sizeof(*out));
+  for (int k = 0; k < 2; k++) {
+    int hd, tl;
+    compute_split(h[k].level_ptr, nl[k], serial_width, hd, tl);
+    int tasks = 0;
+    for (int l = hd; l < tl; l++) tasks += (h[k].level_ptr[l + 1] - h[k].level_ptr[l] + kTaskRows - 1) / kTaskRows;
+    out->nlevels[k] = nl[k];
+    out->head_end[k] = hd;
+    out->tail_begin[k] = tl;
+    out->mid_tasks[k] = tasks;
+    out->level_ptr[k] = (int *)malloc((nl[k] + 1) * sizeof(int));
+    out->slot_row[k] = (int *)malloc(n * sizeof(int));
+    if (!out->level_ptr[k] || !out->slot_row[k]) return fail(RCHOL_AMD_EINVAL, "out of memory");
+    memcpy(out->level_ptr[k], h[k].level_ptr.data(), (nl[k] + 1) * sizeof(int));
+    memcpy(out->slot_row[k], h[k].slot_row.data(), n * sizeof(int));
+  }
+  return RCHOL_AMD_OK;
+}
+
+void rchol_amd_schedule_free(rchol_amd_schedule *s) {
+  if (!s) return;
+  for (int k = 0; k < 2; k++) {
+    free(s->level_ptr[k]);
+    free(s->slot_row[k]);
+    s->level_ptr[k] = nullptr;
+    s->slot_row[k] = nullptr;
+  }
 }
 
 int rchol_amd_pcg_csr(size_t n, const size_t *aRowPtr, const size_t *aColIdx, const double *aVal,

@@ -298,6 +298,33 @@ def stack_systems(systems):
     return A, G, np.asarray(off, dtype=np.uint64)
 
 
+def analyze(G, serial_width=0):
+    """Host-only level schedule of the two sweeps for a factor (stored triple
+    or scipy lower G): dict per sweep with levels, split and slot order."""
+    if isinstance(G, tuple):
+        cp, ri, gv = G
+    else:
+        cp, ri, gv = stored_from_lower(G)
+    cp, pcp = _u64(cp)
+    ri, pri = _u64(ri)
+    gv, pgv = _f64(gv)
+    n = cp.size - 1
+    sc = _lib.Schedule()
+    lib = _lib.engine()
+    _lib.check(lib.rchol_amd_analyze_G(n, pcp, pri, pgv, int(serial_width), ctypes.byref(sc)))
+    out = {}
+    for k, name in enumerate(("fwd", "bwd")):
+        nl = sc.nlevels[k]
+        out[name] = {
+            "nlevels": nl, "head_end": sc.head_end[k], "tail_begin": sc.tail_begin[k],
+            "mid_tasks": sc.mid_tasks[k],
+            "level_ptr": np.ctypeslib.as_array(sc.level_ptr[k], shape=(nl + 1,)).copy(),
+            "slot_row": np.ctypeslib.as_array(sc.slot_row[k], shape=(n,)).copy(),
+        }
+    lib.rchol_amd_schedule_free(ctypes.byref(sc))
+    return out
+
+
 def pcg(A, b, epsilon, maxitr, G, Gt=None):
     """Drop-in for python/rchol/util.py:32-61: pcg(A, b, epsilon, maxitr, G, Gt).
 

@@ -1,0 +1,75 @@
+"""Host-side level schedule of the triangular sweeps (CPU, no device).
+
+The persistent sweep kernel is deadlock-free only if every dependency of a
+row lies in a strictly lower level of its sweep and the schedule covers every
+row exactly once; the split head/tail phases need levels of at most the
+threshold width.  Checked on the reference fixtures and on this build's
+factors through rchol_amd_analyze_G (the code upload_G runs)."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import rchol_amd
+from rchol_amd import factor as F
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+FIXTURES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLD, "*.npz")))
+
+
+def factors():
+    out = []
+    for name in FIXTURES:
+        d = np.load(os.path.join(GOLD, name + ".npz"))
+        out.append((name, (d["g_colptr"], d["g_rowidx"], d["g_val"])))
+    A = F.varcoef_3d(14, 13, 12, 1.5, 1e-3, 3)
+    perm, blocks = F.nd_grid(14, 13, 12, 4)
+    for order in ("static", "mis"):
+        Gs, _, _ = F.factor(A, perm, blocks, order, 2)
+        out.append((f"var14_{order}", Gs))
+    return out
+
+
+CASES = factors()
+
+
+@pytest.mark.parametrize("name,Gs", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("width", [0, 8, 64])
+def test_schedule_respects_dependencies(name, Gs, width):
+    cp, ri, _ = (np.asarray(a) for a in Gs)
+    n = cp.size - 1
+    sch = rchol_amd.analyze(Gs, width)
+    cols = np.repeat(np.arange(n), np.diff(cp.astype(np.int64)))
+    rows = ri.astype(np.int64)
+    off = rows != cols
+    for k, key in enumerate(("fwd", "bwd")):
+        s = sch[key]
+        lp, sr = s["level_ptr"], s["slot_row"]
+        assert lp[0] == 0 and lp[-1] == n and np.all(np.diff(lp) > 0)
+        assert np.array_equal(np.sort(sr), np.arange(n))
+        lev = np.empty(n, dtype=np.int64)
+        lev[sr] = np.repeat(np.arange(s["nlevels"]), np.diff(lp))
+        if key == "fwd":   # row i waits for column j < i
+            assert np.all(lev[rows[off]] > lev[cols[off]])
+        else:              # row j of G^T waits for i > j
+            assert np.all(lev[cols[off]] > lev[rows[off]])
+        widths = np.diff(lp)
+        h, t = s["head_end"], s["tail_begin"]
+        assert 0 <= h <= t <= s["nlevels"]
+        if width == 0:
+            assert h == 0 and t == s["nlevels"]
+        else:
+            assert np.all(widths[:h] <= width) and np.all(widths[t:] <= width)
+            assert h == s["nlevels"] or widths[h] > width
+        assert s["mid_tasks"] == int(np.sum((widths[h:t] + 63) // 64))
+
+
+def test_schedule_rejects_invalid_factors():
+    with pytest.raises(RuntimeError, match="error -1"):
+        rchol_amd.analyze((np.array([0, 1, 3]), np.array([0, 0, 1]), np.ones(3)))  # upper entry
+    cp = np.array([0, 2, 3])
+    with pytest.raises(RuntimeError, match="error -5"):
+        rchol_amd.analyze((cp, np.array([0, 1, 1]), np.array([0.0, 1.0, 1.0])))  # zero pivot
+    with pytest.raises(RuntimeError, match="error -1"):
+        rchol_amd.analyze((cp, np.array([0, 0, 1]), np.ones(3)))  # two diagonals
