@@ -46,9 +46,22 @@ __global__ __launch_bounds__(256) void k_sptrsv(
   double dacc = 0.0;
   for (int t = wave; t < ntasks; t += nwaves) {
     const int s0 = task_ptr[t], s1 = task_ptr[t + 1];
-    for (int s = s0 + grp; s < s1; s += R) {
-      const int row = slot_row[s];
-      const int e0 = rp[s], e1 = rp[s + 1];
+    const int cnt = s1 - s0;  // <= 64 slots, one coalesced load of their metadata
+    int m_row = 0, m_e0 = 0, m_e1 = 0;
+    VT m_dg = 1;
+    if (lane < cnt) {
+      m_row = slot_row[s0 + lane];
+      m_e0 = rp[s0 + lane];
+      m_e1 = rp[s0 + lane + 1];
+      m_dg = diag[s0 + lane];
+    }
+    for (int base = 0; base < cnt; base += R) {
+      const int k = base + grp;
+      const int row = __shfl(m_row, k, 64);
+      const int e0 = __shfl(m_e0, k, 64), e1 = __shfl(m_e1, k, 64);
+      const VT dg = __shfl(m_dg, k, 64);
+      if (k >= cnt) continue;
+      const BT b = (sub == 0) ? rhs[row] : (BT)0;
       XT acc = 0;
       for (int e = e0 + sub; e < e1; e += L) {
         const int j = ci[e];
@@ -57,7 +70,7 @@ __global__ __launch_bounds__(256) void k_sptrsv(
       }
       acc = group_sum<L>(acc);
       if (sub == 0) {
-        const XT v = ((XT)rhs[row] - acc) / (XT)diag[s];
+        const XT v = ((XT)b - acc) / (XT)dg;
         Sent<XT>::store_agent(x + row, v);
         if (FUSE == 1) aux[row] = Sent<XT>::value();
         if (FUSE == 2) dacc += dotv[row] * (double)v;
