@@ -103,17 +103,31 @@ __global__ __launch_bounds__(256) void k_sptrsv_mr(
   double dacc = 0.0;
   for (int t = wave; t < ntasks; t += nwaves) {
     const int s0 = task_ptr[t], s1 = task_ptr[t + 1];
-    for (int s = s0 + grp; s < s1; s += R) {
-      const int row = slot_row[s];
-      const int e0 = rp[s], e1 = rp[s + 1];
+    const int cnt = s1 - s0;  // <= 64 slots: one coalesced metadata load
+    int m_row = 0, m_e0 = 0, m_e1 = 0;
+    VT m_dg = 1;
+    if (lane < cnt) {
+      m_row = slot_row[s0 + lane];
+      m_e0 = rp[s0 + lane];
+      m_e1 = rp[s0 + lane + 1];
+      m_dg = diag[s0 + lane];
+    }
+    for (int base = 0; base < cnt; base += R) {
+      const int kk = base + grp;
+      const int row = __shfl(m_row, kk, 64);
+      const int e0 = __shfl(m_e0, kk, 64), e1 = __shfl(m_e1, kk, 64);
+      const VT dg = __shfl(m_dg, kk, 64);
+      if (kk >= cnt) continue;
+      const size_t o = (size_t)row * K + c;
+      const BT b = rhs[o];
       XT acc = 0;
+#pragma unroll 4
       for (int e = e0; e < e1; e++) {
         const int j = ci[e];
         const XT xj = wait_ready<XT>(x + (size_t)j * K + c, err);
         acc += (XT)cv[e] * xj;
       }
-      const size_t o = (size_t)row * K + c;
-      const XT v = ((XT)rhs[o] - acc) / (XT)diag[s];
+      const XT v = ((XT)b - acc) / (XT)dg;
       Sent<XT>::store_agent(x + o, v);
       if (FUSE == 1) aux[o] = Sent<XT>::value();
       if (FUSE == 2) dacc += dotv[o] * (double)v;
