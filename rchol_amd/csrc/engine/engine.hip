@@ -511,7 +511,7 @@ int rchol_amd_ctx::pcg(const double *d_b, double tol, int maxit, double *d_x_out
   }
   for (int i = 0; i < 4; i++) stats.kernel_ms[i] = 0.0;
   stats.kernel_iters = 0;
-  if (graph_iters > 0 && !gexec && !(profile && profile_no_graph)) {
+  if (graph_iters > 0 && !gexec && !profile) {
     HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < K; i++) launch_iteration(i & 1, i);
     hipError_t e = hipStreamEndCapture(stream, &graph);
@@ -770,7 +770,7 @@ int rchol_amd_ctx::pcg_batch(const double *d_b, int ncols, double tol, int maxit
   }
   for (int i = 0; i < 4; i++) stats.kernel_ms[i] = 0.0;
   stats.kernel_iters = 0;
-  if (graph_iters > 0 && !batch.gexec && !(profile && profile_no_graph)) {
+  if (graph_iters > 0 && !batch.gexec && !profile) {
     HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < G; i++) launch_iteration_mr<K>(i & 1, i);
     hipError_t e = hipStreamEndCapture(stream, &batch.graph);
@@ -983,7 +983,7 @@ int rchol_amd_ctx::pcg_segments(const double *d_b, double tol, int maxit, double
   }
   for (int i = 0; i < 4; i++) stats.kernel_ms[i] = 0.0;
   stats.kernel_iters = 0;
-  if (graph_iters > 0 && !segs.gexec && !(profile && profile_no_graph)) {
+  if (graph_iters > 0 && !segs.gexec && !profile) {
     HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < G; i++) launch_iteration_seg(i & 1, i);
     hipError_t e = hipStreamEndCapture(stream, &segs.graph);
