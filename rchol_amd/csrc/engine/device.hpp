@@ -72,24 +72,49 @@ template <> struct Sent<float> {
 // flag is up every other wait returns at its next check.
 constexpr long long kWaitTicks = 1ll << 26;
 
-// Value of x[j] once its producer has published it.
+// Value of x[j] once its producer has published it.  Wave-cooperative: all
+// active lanes read their value once; while some still see the sentinel,
+// only the first such lane polls (agent-scope load, exponential back-off up
+// to ~2k clocks) and the others re-read once it has seen its value.  This
+// keeps the polling traffic of thousands of resident waves to one load per
+// waiting wave instead of one per waiting lane.  The error flag is read
+// only on the slow path, and a wait that outlives kWaitTicks raises it.
 template <typename T>
 __device__ __forceinline__ T wait_ready(const T *p, int *err) {
   T v = *p;
-  if (!Sent<T>::is(v)) return v;
+  bool pending = Sent<T>::is(v);
+  unsigned long long mask = __ballot(pending);
+  if (!mask) return v;
+  const int lane = __lane_id();
   const long long t0 = wall_clock64();
-  unsigned int polls = 0;
-  do {
-    __builtin_amdgcn_s_sleep(2);
-    v = Sent<T>::load_agent(p);
-    if ((++polls & 127u) == 0) {
-      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-      if (wall_clock64() - t0 > kWaitTicks) {
-        atomicOr(err, 1);
-        break;
+  int backoff = 1;
+  while (mask) {
+    const int leader = __ffsll((long long)mask) - 1;
+    if (lane == leader) {
+      for (;;) {
+        for (int i = 0; i < backoff; i++) __builtin_amdgcn_s_sleep(2);
+        v = Sent<T>::load_agent(p);
+        if (!Sent<T>::is(v)) break;
+        if (backoff < 16) backoff <<= 1;
+        if (wall_clock64() - t0 > kWaitTicks ||
+            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          atomicOr(err, 1);
+          v = T(0);
+          break;
+        }
       }
+      pending = false;
+    } else if (pending) {
+      v = Sent<T>::load_agent(p);
+      pending = Sent<T>::is(v);
     }
-  } while (Sent<T>::is(v));
+    mask = __ballot(pending);
+    // a raised error releases every waiter
+    if (mask && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+      if (pending) v = T(0);
+      break;
+    }
+  }
   return v;
 }
 
