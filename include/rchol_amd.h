@@ -45,6 +45,7 @@ typedef struct rchol_amd_ctx rchol_amd_ctx;
 #define RCHOL_AMD_OPT_PROFILE 5       /* 1: time every kernel of the loop with HIP events */
 #define RCHOL_AMD_OPT_SERIAL_WIDTH 6  /* leading/trailing sweep levels of at most this many
                                          rows run in one workgroup (0 = off) */
+#define RCHOL_AMD_OPT_COHERENT_GATHER 7 /* 1: sweeps gather x with agent-scope loads */
 
 /* Counters filled by rchol_amd_get_stats. */
 typedef struct rchol_amd_stats {

@@ -80,10 +80,18 @@ constexpr long long kWaitTicks = 1ll << 26;
 // waiting wave instead of one per waiting lane.  The error flag is read
 // only on the slow path, and a wait that outlives kWaitTicks raises it.
 template <typename T>
-__device__ __forceinline__ T wait_ready(const T *p, int *err) {
-  T v = *p;
+__device__ __forceinline__ T wait_ready(const T *p, int *err, int coherent = 0) {
+  T v = coherent ? Sent<T>::load_agent(p) : *p;
   bool pending = Sent<T>::is(v);
   unsigned long long mask = __ballot(pending);
+  if (!mask) return v;
+  // a plain load may have hit a line cached before the value was published:
+  // re-read once, coherently, before anybody sleeps
+  if (pending && !coherent) {
+    v = Sent<T>::load_agent(p);
+    pending = Sent<T>::is(v);
+  }
+  mask = __ballot(pending);
   if (!mask) return v;
   const int lane = __lane_id();
   const long long t0 = wall_clock64();

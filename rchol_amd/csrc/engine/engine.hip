@@ -182,6 +182,7 @@ struct rchol_amd_ctx {
   int profile = 0;
   int profile_no_graph = 0;
   int serial_width = 0;  // narrow-level threshold of the split sweeps (0 = off)
+  int coherent_gather = 0;  // sweeps read x with agent-scope loads from the start
   std::vector<hipEvent_t> pev;  // 5 per captured iteration when profiling
   // captured iteration graph
   hipGraph_t graph = nullptr;
@@ -299,7 +300,7 @@ void launch_one_sweep(rchol_amd_ctx *c, const Sweep &sw, const VT *cv, const VT 
     hipLaunchKernelGGL((k_sptrsv<VT, XT, BT, L, FUSE>), dim3(c->trsv_grid), dim3(kBlock), 0,
                        c->stream, sw.mid_ntasks, sw.mid_task_ptr.p, sw.slot_row.p, sw.rp.p,
                        sw.ci.p, cv, dg, rhs, x, aux, dotv, c->partials.p, c->sc, stop,
-                       tail ? 1 : 0);
+                       tail ? 1 : 0, c->coherent_gather);
   if (tail)
     hipLaunchKernelGGL((k_sptrsv_serial<VT, XT, BT, L, FUSE>), dim3(1), dim3(1024), 0, c->stream,
                        sw.tail_begin, sw.nlevels, sw.level_ptr.p, sw.slot_row.p, sw.rp.p, sw.ci.p,
@@ -1310,6 +1311,9 @@ int rchol_amd_set_option(rchol_amd_ctx *c, int key, double value) {
       break;
     case RCHOL_AMD_OPT_PROFILE:
       c->profile = v ? 1 : 0;
+      break;
+    case RCHOL_AMD_OPT_COHERENT_GATHER:
+      c->coherent_gather = v ? 1 : 0;
       break;
     case RCHOL_AMD_OPT_SERIAL_WIDTH:
       if (v < 0 || v > 1 << 20) return fail(RCHOL_AMD_EINVAL, "serial width %d out of range", v);

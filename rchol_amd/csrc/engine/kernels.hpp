@@ -35,7 +35,7 @@ __global__ __launch_bounds__(256) void k_sptrsv(
     const int *__restrict__ rp, const int *__restrict__ ci, const VT *__restrict__ cv,
     const VT *__restrict__ diag, const BT *__restrict__ rhs, XT *x, XT *__restrict__ aux,
     const double *__restrict__ dotv, double *__restrict__ partials, DevScalars *sc,
-    const int *stop, int defer) {
+    const int *stop, int defer, int coherent) {
   if (*stop) return;
   constexpr int R = 64 / L;
   const int lane = threadIdx.x & 63;
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void k_sptrsv(
       XT acc = 0;
       for (int e = e0 + sub; e < e1; e += L) {
         const int j = ci[e];
-        const XT xj = wait_ready<XT>(x + j, err);
+        const XT xj = wait_ready<XT>(x + j, err, coherent);
         acc += (XT)cv[e] * xj;
       }
       acc = group_sum<L>(acc);

@@ -20,6 +20,7 @@ OPT_SPMV_LANES = 3
 OPT_TRSV_LANES = 4
 OPT_PROFILE = 5
 OPT_SERIAL_WIDTH = 6
+OPT_COHERENT_GATHER = 7
 
 
 class Stats(ctypes.Structure):
