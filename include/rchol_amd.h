@@ -41,7 +41,10 @@ typedef struct rchol_amd_ctx rchol_amd_ctx;
 #define RCHOL_AMD_OPT_GRAPH_ITERS 1   /* PCG iterations per captured hipGraph (0 = no graph) */
 #define RCHOL_AMD_OPT_RESID_REPLACE 2 /* true-residual replacement period (0 = never) */
 #define RCHOL_AMD_OPT_SPMV_LANES 3    /* lanes per row in SpMV (4, 8, 16 or 32) */
-#define RCHOL_AMD_OPT_TRSV_LANES 4    /* lanes per row in the triangular sweeps */
+#define RCHOL_AMD_OPT_TRSV_LANES 4    /* minimum lanes per row in the triangular sweeps
+                                         (power of two <= 64, default 1); a task
+                                         widens to cover its longest row in one
+                                         round of 4 gathers per lane */
 #define RCHOL_AMD_OPT_PROFILE 5       /* 1: time every kernel of the loop with HIP events */
 #define RCHOL_AMD_OPT_SERIAL_WIDTH 6  /* leading/trailing sweep levels of at most this many
                                          rows run in one workgroup (0 = off) */
@@ -73,7 +76,8 @@ typedef struct rchol_amd_schedule {
   int nlevels[2];    /* [0] forward G y = r, [1] backward G^T z = y */
   int head_end[2];   /* levels [0, head_end) run in one workgroup */
   int tail_begin[2]; /* levels [tail_begin, nlevels) run in one workgroup */
-  int mid_tasks[2];  /* tasks of the persistent kernel in between */
+  int mid_tasks[2];  /* 64-slot chunks of the levels in between (the engine
+                        cuts narrow levels finer, by its resident waves) */
   int *level_ptr[2]; /* nlevels+1 slot offsets per level (malloc'ed) */
   int *slot_row[2];  /* n: row held by each slot, in level order (malloc'ed) */
 } rchol_amd_schedule;
@@ -177,10 +181,11 @@ int rchol_amd_edge_grad_device(rchol_amd_ctx *ctx, size_t m, const long long *d_
                                const long long *d_v, const double *d_lam, const double *d_x,
                                double *d_g);
 
-/* Autotuner: times the preconditioner application and the SpMV with 4, 8
- * and 16 lanes per row on the uploaded matrices (median of reps runs) and
- * keeps the fastest (RCHOL_AMD_OPT_TRSV_LANES / _SPMV_LANES).  Outputs may
- * be NULL.  No reference counterpart. */
+/* Autotuner: times the preconditioner application with a minimum of 1, 2,
+ * 4 and 8 lanes per sweep row and the SpMV with 4, 8 and 16 lanes per row on
+ * the uploaded matrices (median of reps runs) and keeps the fastest
+ * (RCHOL_AMD_OPT_TRSV_LANES / _SPMV_LANES).  Outputs may be NULL.  No
+ * reference counterpart. */
 int rchol_amd_autotune(rchol_amd_ctx *ctx, int reps, int *trsv_lanes, int *spmv_lanes,
                        double *trsv_ms, double *spmv_ms);
 

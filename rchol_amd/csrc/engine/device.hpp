@@ -126,6 +126,72 @@ __device__ __forceinline__ T wait_ready(const T *p, int *err, int coherent = 0) 
   return v;
 }
 
+// Back-off cap of the polling lane of await_batch, in s_sleep(1) units
+// (64 clocks each): bounds the extra delay between publication and
+// observation to ~kPollCap * 64 clocks.
+#ifndef RCHOL_AMD_POLL_CAP
+#define RCHOL_AMD_POLL_CAP 8
+#endif
+constexpr int kPollCap = RCHOL_AMD_POLL_CAP;
+
+// Completes a round of U gathers per lane, xv[u] = x[j[u]] (j[u] < 0: an
+// unused slot), once some lane of the wave still holds a sentinel.  Called
+// by the whole wave, converged.  Pending slots are re-read coherently first
+// (a plain load may have hit a line cached before publication); while some
+// remain, one lane polls the first pending address of the wave with
+// back-off, and once that value is there every lane re-reads all of its
+// pending slots at once (one load latency for the whole round).  A wait
+// longer than kWaitTicks raises the error flag; a raised flag zeroes what is
+// still pending and returns.
+template <typename T, int U>
+__device__ __forceinline__ void await_batch(const T *x, const int (&j)[U], T (&xv)[U], int *err) {
+  bool pend = false;
+#pragma unroll
+  for (int u = 0; u < U; u++)
+    if (j[u] >= 0 && Sent<T>::is(xv[u])) xv[u] = Sent<T>::load_agent(x + j[u]);
+#pragma unroll
+  for (int u = 0; u < U; u++) pend |= j[u] >= 0 && Sent<T>::is(xv[u]);
+  unsigned long long mask = __ballot(pend);
+  if (!mask) return;
+  const int lane = __lane_id();
+  const long long t0 = wall_clock64();
+  int backoff = 1;
+  while (mask) {
+    const int leader = __ffsll((long long)mask) - 1;
+    int jp = -1;
+#pragma unroll
+    for (int u = U - 1; u >= 0; u--)
+      if (j[u] >= 0 && Sent<T>::is(xv[u])) jp = j[u];
+    jp = __shfl(jp, leader, 64);
+    if (lane == leader) {
+      for (int polls = 1;; polls++) {
+        for (int i = 0; i < backoff; i++) __builtin_amdgcn_s_sleep(1);
+        if (!Sent<T>::is(Sent<T>::load_agent(x + jp))) break;
+        if (backoff < kPollCap) backoff <<= 1;
+        if ((polls & 7) == 0 &&
+            (wall_clock64() - t0 > kWaitTicks ||
+             __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+          atomicOr(err, 1);
+          break;
+        }
+      }
+    }
+    pend = false;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (j[u] >= 0 && Sent<T>::is(xv[u])) xv[u] = Sent<T>::load_agent(x + j[u]);
+#pragma unroll
+    for (int u = 0; u < U; u++) pend |= j[u] >= 0 && Sent<T>::is(xv[u]);
+    mask = __ballot(pend);
+    if (mask && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (j[u] >= 0 && Sent<T>::is(xv[u])) xv[u] = T(0);
+      break;
+    }
+  }
+}
+
 // Sum over the L consecutive lanes of a row group (L a power of two <= 64).
 template <int L>
 __device__ __forceinline__ double group_sum(double v) {

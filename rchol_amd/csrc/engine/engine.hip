@@ -81,8 +81,8 @@ struct Sweep {
   DevBuf<int> task_ptr, slot_row, rp, ci;
   // split schedule: levels [0, head_end) and [tail_begin, nlevels) run in one
   // workgroup (k_sptrsv_serial), the middle in the persistent kernel
-  std::vector<int> level_ptr_h;
-  DevBuf<int> level_ptr, mid_task_ptr;
+  std::vector<int> level_ptr_h, rp_h;
+  DevBuf<int> level_ptr, level_lg, mid_task_ptr, mid_task_lg;
   int head_end = 0, tail_begin = 0, mid_ntasks = 0;
   DevBuf<double> cv64, diag64;
   DevBuf<float> cv32, diag32;
@@ -95,16 +95,27 @@ struct HostSweep {
   int nlevels = 0;
 };
 
-// Counting sort of rows by level, then level-chunked tasks.
-void build_slots(const std::vector<int> &lev, int nlev, HostSweep &hs, std::vector<int> &slot_of) {
+// Rows sorted by level and, inside a level, by decreasing length (two
+// stable counting sorts), then level-chunked 64-slot tasks.
+void build_slots(const std::vector<int> &lev, int nlev, const std::vector<int> &len,
+                 HostSweep &hs, std::vector<int> &slot_of) {
   const int n = (int)lev.size();
+  int maxlen = 0;
+  for (int i = 0; i < n; i++) maxlen = std::max(maxlen, len[i]);
+  std::vector<int> by_len(n);
+  {
+    std::vector<int> cl(maxlen + 2, 0);
+    for (int i = 0; i < n; i++) cl[maxlen - len[i] + 1]++;
+    for (int k = 0; k <= maxlen; k++) cl[k + 1] += cl[k];
+    for (int i = 0; i < n; i++) by_len[cl[maxlen - len[i]]++] = i;
+  }
   std::vector<int> cnt(nlev + 1, 0);
   for (int i = 0; i < n; i++) cnt[lev[i] + 1]++;
   for (int l = 0; l < nlev; l++) cnt[l + 1] += cnt[l];
   hs.slot_row.assign(n, 0);
   slot_of.assign(n, 0);
   std::vector<int> cur(cnt.begin(), cnt.end() - 1);
-  for (int i = 0; i < n; i++) {
+  for (int i : by_len) {
     int s = cur[lev[i]]++;
     hs.slot_row[s] = i;
     slot_of[i] = s;
@@ -119,8 +130,7 @@ void build_slots(const std::vector<int> &lev, int nlev, HostSweep &hs, std::vect
 }
 
 // Cut a sweep's levels into a narrow head, a middle and a narrow tail
-// (levels of at most `width` rows run in one workgroup; width 0 = no split)
-// and upload the middle's task list.
+// (levels of at most `width` rows run in one workgroup; width 0 = no split).
 void compute_split(const std::vector<int> &lp, int L, int width, int &h, int &t) {
   h = 0;
   t = L;
@@ -130,18 +140,58 @@ void compute_split(const std::vector<int> &lp, int L, int width, int &h, int &t)
   }
 }
 
-hipError_t split_levels(Sweep &sw, int width) {
+// Lane width (log2) for a row of len entries: the smallest width >= the
+// minimum that covers the row in one round of kGather gathers per lane.
+int lane_lg(int len, int lmin_lg) {
+  int lg = lmin_lg;
+  while (lg < 6 && (kGather << lg) < len) lg++;
+  return lg;
+}
+
+// Tasks of the middle levels [h, t).  Slots of a level are sorted by
+// decreasing length, so a task's lane width is set by its first row.  A
+// task holds at most 64 slots: as many row passes (64/L rows each) as it
+// takes for the level to need about one task per wave, so narrow levels
+// spread over the waves and wide ones amortise the per-task overhead.
+void cut_tasks(const std::vector<int> &lp, const std::vector<int> &rp, int h, int t, int lmin_lg,
+               long long nwaves, std::vector<int> &ptr, std::vector<int> &lgs) {
+  ptr.clear();
+  lgs.clear();
+  for (int l = h; l < t; l++) {
+    const int s_end = lp[l + 1];
+    const long long rows = s_end - lp[l];
+    for (int s = lp[l]; s < s_end;) {
+      const int lg = lane_lg(rp[s + 1] - rp[s], lmin_lg);
+      const int R = 64 >> lg;
+      const long long one_pass = (rows + R - 1) / R;
+      const int passes =
+          (int)std::min<long long>(64 / R, std::max<long long>(1, (one_pass + nwaves - 1) / nwaves));
+      ptr.push_back(s);
+      lgs.push_back(lg);
+      s += std::min(s_end - s, passes * R);
+    }
+  }
+  ptr.push_back(t > h ? lp[t] : lp[h]);
+}
+
+// Upload a sweep's split schedule: head/tail bounds, per-level lane widths
+// (for the one-workgroup phases) and the middle's task list.
+hipError_t split_levels(Sweep &sw, int width, int lmin_lg, long long nwaves) {
   const std::vector<int> &lp = sw.level_ptr_h;
   int h, t;
   compute_split(lp, sw.nlevels, width, h, t);
-  std::vector<int> tasks;
-  for (int l = h; l < t; l++)
-    for (int s = lp[l]; s < lp[l + 1]; s += kTaskRows) tasks.push_back(s);
-  tasks.push_back(t > h ? lp[t] : lp[h]);
+  std::vector<int> tasks, lgs, llg(std::max(sw.nlevels, 1), lmin_lg);
+  for (int l = 0; l < sw.nlevels; l++)
+    if (lp[l + 1] > lp[l]) llg[l] = lane_lg(sw.rp_h[lp[l] + 1] - sw.rp_h[lp[l]], lmin_lg);
+  cut_tasks(lp, sw.rp_h, h, t, lmin_lg, nwaves, tasks, lgs);
   sw.head_end = h;
   sw.tail_begin = t;
   sw.mid_ntasks = (int)tasks.size() - 1;
-  return sw.mid_task_ptr.upload(tasks.data(), tasks.size());
+  if (lgs.empty()) lgs.push_back(lmin_lg);
+  hipError_t e = sw.level_lg.upload(llg.data(), llg.size());
+  if (e == hipSuccess) e = sw.mid_task_lg.upload(lgs.data(), lgs.size());
+  if (e == hipSuccess) e = sw.mid_task_ptr.upload(tasks.data(), tasks.size());
+  return e;
 }
 
 }  // namespace
@@ -178,7 +228,7 @@ struct rchol_amd_ctx {
   int graph_iters = 16;
   int resid_period = 0;
   int spmv_lanes = 8;
-  int trsv_lanes = 8;
+  int trsv_lanes = 1;  // minimum lanes per sweep row (tasks widen to fit)
   int profile = 0;
   int profile_no_graph = 0;
   int serial_width = 0;  // narrow-level threshold of the split sweeps (0 = off)
@@ -288,55 +338,44 @@ namespace {
 // One sweep in up to three launches: serial head, persistent middle, serial
 // tail (the split is empty unless RCHOL_AMD_OPT_SERIAL_WIDTH is set).
 // FUSE 1: sentinel into aux; 2: r.z -> beta; 0: plain.
-template <typename VT, typename XT, typename BT, int L, int FUSE>
+template <typename VT, typename XT, typename BT, int FUSE>
 void launch_one_sweep(rchol_amd_ctx *c, const Sweep &sw, const VT *cv, const VT *dg,
                       const BT *rhs, XT *x, XT *aux, const double *dotv, const int *stop) {
   const bool head = sw.head_end > 0, tail = sw.tail_begin < sw.nlevels;
   if (head)
-    hipLaunchKernelGGL((k_sptrsv_serial<VT, XT, BT, L, FUSE>), dim3(1), dim3(1024), 0, c->stream,
-                       0, sw.head_end, sw.level_ptr.p, sw.slot_row.p, sw.rp.p, sw.ci.p, cv, dg,
-                       rhs, x, aux, dotv, c->sc, stop, 0);
+    hipLaunchKernelGGL((k_sptrsv_serial<VT, XT, BT, FUSE>), dim3(1), dim3(1024), 0, c->stream, 0,
+                       sw.head_end, sw.level_ptr.p, sw.level_lg.p, sw.slot_row.p, sw.rp.p,
+                       sw.ci.p, cv, dg, rhs, x, aux, dotv, c->sc, stop, 0);
   if (sw.mid_ntasks > 0 || (FUSE == 2 && !tail))
-    hipLaunchKernelGGL((k_sptrsv<VT, XT, BT, L, FUSE>), dim3(c->trsv_grid), dim3(kBlock), 0,
-                       c->stream, sw.mid_ntasks, sw.mid_task_ptr.p, sw.slot_row.p, sw.rp.p,
-                       sw.ci.p, cv, dg, rhs, x, aux, dotv, c->partials.p, c->sc, stop,
-                       tail ? 1 : 0, c->coherent_gather);
+    hipLaunchKernelGGL((k_sptrsv<VT, XT, BT, FUSE>), dim3(c->trsv_grid), dim3(kBlock), 0,
+                       c->stream, sw.mid_ntasks, sw.mid_task_ptr.p, sw.mid_task_lg.p,
+                       sw.slot_row.p, sw.rp.p, sw.ci.p, cv, dg, rhs, x, aux, dotv, c->partials.p,
+                       c->sc, stop, tail ? 1 : 0, c->coherent_gather);
   if (tail)
-    hipLaunchKernelGGL((k_sptrsv_serial<VT, XT, BT, L, FUSE>), dim3(1), dim3(1024), 0, c->stream,
-                       sw.tail_begin, sw.nlevels, sw.level_ptr.p, sw.slot_row.p, sw.rp.p, sw.ci.p,
-                       cv, dg, rhs, x, aux, dotv, c->sc, stop, 1);
+    hipLaunchKernelGGL((k_sptrsv_serial<VT, XT, BT, FUSE>), dim3(1), dim3(1024), 0, c->stream,
+                       sw.tail_begin, sw.nlevels, sw.level_ptr.p, sw.level_lg.p, sw.slot_row.p,
+                       sw.rp.p, sw.ci.p, cv, dg, rhs, x, aux, dotv, c->sc, stop, 1);
 }
 
-template <typename VT, typename XT, int L>
-int launch_sweeps_t(rchol_amd_ctx *c, const double *rhs, XT *y, XT *z, const int *stop,
-                    bool with_dot, const double *dotv, hipEvent_t *ev) {
+template <typename VT, typename XT>
+int dispatch_sweeps(rchol_amd_ctx *c, const double *rhs, XT *y, XT *z, const int *stop,
+                    bool with_dot, const double *dotv, hipEvent_t *ev = nullptr) {
   const VT *fcv, *fdg, *bcv, *bdg;
   if constexpr (sizeof(VT) == 8) {
     fcv = c->fwd.cv64.p; fdg = c->fwd.diag64.p; bcv = c->bwd.cv64.p; bdg = c->bwd.diag64.p;
   } else {
     fcv = c->fwd.cv32.p; fdg = c->fwd.diag32.p; bcv = c->bwd.cv32.p; bdg = c->bwd.diag32.p;
   }
-  launch_one_sweep<VT, XT, double, L, 1>(c, c->fwd, fcv, fdg, rhs, y, z, nullptr, stop);
+  launch_one_sweep<VT, XT, double, 1>(c, c->fwd, fcv, fdg, rhs, y, z, nullptr, stop);
   if (ev) (void)hipEventRecord(ev[1], c->stream);
   if (with_dot)
-    launch_one_sweep<VT, XT, XT, L, 2>(c, c->bwd, bcv, bdg, (const XT *)y, z, (XT *)nullptr, dotv,
-                                       stop);
+    launch_one_sweep<VT, XT, XT, 2>(c, c->bwd, bcv, bdg, (const XT *)y, z, (XT *)nullptr, dotv,
+                                    stop);
   else
-    launch_one_sweep<VT, XT, XT, L, 0>(c, c->bwd, bcv, bdg, (const XT *)y, z, (XT *)nullptr,
-                                       nullptr, stop);
+    launch_one_sweep<VT, XT, XT, 0>(c, c->bwd, bcv, bdg, (const XT *)y, z, (XT *)nullptr, nullptr,
+                                    stop);
   if (ev) (void)hipEventRecord(ev[2], c->stream);
   return RCHOL_AMD_OK;
-}
-
-template <typename VT, typename XT>
-int dispatch_sweeps(rchol_amd_ctx *c, const double *rhs, XT *y, XT *z, const int *stop,
-                    bool with_dot, const double *dotv, hipEvent_t *ev = nullptr) {
-  switch (c->trsv_lanes) {
-    case 4: return launch_sweeps_t<VT, XT, 4>(c, rhs, y, z, stop, with_dot, dotv, ev);
-    case 16: return launch_sweeps_t<VT, XT, 16>(c, rhs, y, z, stop, with_dot, dotv, ev);
-    case 32: return launch_sweeps_t<VT, XT, 32>(c, rhs, y, z, stop, with_dot, dotv, ev);
-    default: return launch_sweeps_t<VT, XT, 8>(c, rhs, y, z, stop, with_dot, dotv, ev);
-  }
 }
 
 template <typename ZT, int L>
@@ -391,38 +430,37 @@ void launch_spmv_res(rchol_amd_ctx *c, const double *x, const double *b, double 
 
 // Persistent sweep grid: every block must be resident at once (the
 // deadlock-freedom argument of k_sptrsv).  The occupancy is the minimum
-// over the three epilogue variants at the configured lane count.
-template <typename VT, typename XT, int L>
+// over the three epilogue variants.
+template <typename VT, typename XT>
 static int trsv_occupancy() {
   int a = 0, b = 0, c = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, (const void *)k_sptrsv<VT, XT, double, L, 1>,
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, (const void *)k_sptrsv<VT, XT, double, 1>,
                                                    kBlock, 0) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void *)k_sptrsv<VT, XT, XT, L, 2>,
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void *)k_sptrsv<VT, XT, XT, 2>,
                                                    kBlock, 0) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, (const void *)k_sptrsv<VT, XT, XT, L, 0>,
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, (const void *)k_sptrsv<VT, XT, XT, 0>,
                                                    kBlock, 0) != hipSuccess)
     return 0;
   return std::min(a, std::min(b, c));
 }
 
-template <typename VT, typename XT>
-static int trsv_occupancy_l(int lanes) {
-  switch (lanes) {
-    case 4: return trsv_occupancy<VT, XT, 4>();
-    case 16: return trsv_occupancy<VT, XT, 16>();
-    case 32: return trsv_occupancy<VT, XT, 32>();
-    default: return trsv_occupancy<VT, XT, 8>();
-  }
-}
-
+// Grid of the persistent sweeps and the task cut that goes with it: the
+// resident capacity fixes how finely narrow levels are split.
 int rchol_amd_ctx::size_trsv_grid() {
-  int per_cu = gprec == RCHOL_AMD_FP32 ? trsv_occupancy_l<float, float>(trsv_lanes)
-                                       : trsv_occupancy_l<double, double>(trsv_lanes);
+  int per_cu = gprec == RCHOL_AMD_FP32 ? trsv_occupancy<float, float>()
+                                       : trsv_occupancy<double, double>();
   if (per_cu <= 0) return fail(RCHOL_AMD_EHIP, "occupancy query for the sweep kernels failed");
   per_cu = std::min(per_cu, 8);
-  const int max_tasks = std::max(fwd.ntasks, bwd.ntasks);
+  const long long cap_waves = (long long)per_cu * ncu * (kBlock / 64);
+  int lmin_lg = 0;
+  while ((1 << lmin_lg) < trsv_lanes) lmin_lg++;
+  HIP_TRY(split_levels(fwd, serial_width, lmin_lg, cap_waves));
+  HIP_TRY(split_levels(bwd, serial_width, lmin_lg, cap_waves));
+  const int max_tasks = std::max(fwd.mid_ntasks, bwd.mid_ntasks);
   trsv_grid = std::max(1, std::min(per_cu * ncu, (max_tasks + 3) / 4));
   stats.trsv_waves = trsv_grid * (kBlock / 64);
+  stats.fwd_tasks = fwd.mid_ntasks;
+  stats.bwd_tasks = bwd.mid_ntasks;
   return RCHOL_AMD_OK;
 }
 
@@ -1178,7 +1216,7 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
   }
   std::vector<int> slot_of;
   // forward sweep: CSR of lower G in slot order
-  build_slots(levf, nlf, hf, slot_of);
+  build_slots(levf, nlf, offf, hf, slot_of);
   hf.rp.assign(n + 1, 0);
   for (size_t s = 0; s < n; s++) hf.rp[s + 1] = hf.rp[s] + offf[hf.slot_row[s]];
   const size_t noff = nnz - n;
@@ -1200,7 +1238,11 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
     for (size_t s = 0; s < n; s++) hf.diag[s] = diag[hf.slot_row[s]];
   }
   // backward sweep: the stored rows (CSR of G^T) in slot order
-  build_slots(levb, nlb, hb, slot_of);
+  {
+    std::vector<int> offb(n);
+    for (size_t j = 0; j < n; j++) offb[j] = (int)(colPtr[j + 1] - colPtr[j] - 1);
+    build_slots(levb, nlb, offb, hb, slot_of);
+  }
   hb.rp.assign(n + 1, 0);
   for (size_t s = 0; s < n; s++) {
     int row = hb.slot_row[s];
@@ -1250,8 +1292,8 @@ int rchol_amd_upload_G(rchol_amd_ctx *c, size_t n, const size_t *colPtr, const s
     sw.nnz_off = noff;
     HIP_TRY(sw.task_ptr.upload(h.task_ptr.data(), h.task_ptr.size()));
     sw.level_ptr_h = h.level_ptr;
+    sw.rp_h = h.rp;
     HIP_TRY(sw.level_ptr.upload(h.level_ptr.data(), h.level_ptr.size()));
-    HIP_TRY(split_levels(sw, c->serial_width));
     HIP_TRY(sw.slot_row.upload(h.slot_row.data(), n));
     HIP_TRY(sw.rp.upload(h.rp.data(), n + 1));
     HIP_TRY(sw.ci.upload(h.ci.data(), noff));
@@ -1283,8 +1325,6 @@ int rchol_amd_upload_G(rchol_amd_ctx *c, size_t n, const size_t *colPtr, const s
   c->stats.nnz_g = nnz;
   c->stats.fwd_levels = nlf;
   c->stats.bwd_levels = nlb;
-  c->stats.fwd_tasks = c->fwd.ntasks;
-  c->stats.bwd_tasks = c->bwd.ntasks;
   c->stats.g_precision = precision;
   return RCHOL_AMD_OK;
 }
@@ -1306,7 +1346,8 @@ int rchol_amd_set_option(rchol_amd_ctx *c, int key, double value) {
       c->spmv_lanes = v;
       break;
     case RCHOL_AMD_OPT_TRSV_LANES:
-      if (v != 4 && v != 8 && v != 16 && v != 32) return fail(RCHOL_AMD_EINVAL, "lanes must be 4/8/16/32");
+      if (v < 1 || v > 64 || (v & (v - 1)))
+        return fail(RCHOL_AMD_EINVAL, "sweep lanes must be a power of two in [1, 64]");
       c->trsv_lanes = v;
       break;
     case RCHOL_AMD_OPT_PROFILE:
@@ -1318,10 +1359,6 @@ int rchol_amd_set_option(rchol_amd_ctx *c, int key, double value) {
     case RCHOL_AMD_OPT_SERIAL_WIDTH:
       if (v < 0 || v > 1 << 20) return fail(RCHOL_AMD_EINVAL, "serial width %d out of range", v);
       c->serial_width = v;
-      if (c->hasG) {
-        HIP_TRY(split_levels(c->fwd, v));
-        HIP_TRY(split_levels(c->bwd, v));
-      }
       break;
     default:
       return fail(RCHOL_AMD_EINVAL, "unknown option key %d", key);
@@ -1329,7 +1366,8 @@ int rchol_amd_set_option(rchol_amd_ctx *c, int key, double value) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   c->drop_graph();
-  if (key == RCHOL_AMD_OPT_TRSV_LANES && c->hasG) return c->size_trsv_grid();
+  if ((key == RCHOL_AMD_OPT_TRSV_LANES || key == RCHOL_AMD_OPT_SERIAL_WIDTH) && c->hasG)
+    return c->size_trsv_grid();
   return RCHOL_AMD_OK;
 }
 
@@ -1647,7 +1685,8 @@ int rchol_amd_autotune(rchol_amd_ctx *c, int reps, int *trsv_lanes, int *spmv_la
   std::vector<double> h(n);
   for (size_t i = 0; i < n; i++) h[i] = 1.0 + (double)((i * 2654435761u) % 1000) * 1e-3;
   HIP_TRY(hipMemcpy(c->b.p, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
-  const int cand[3] = {4, 8, 16};
+  const int cand_t[4] = {1, 2, 4, 8};  // minimum lanes per sweep row
+  const int cand_s[3] = {4, 8, 16};    // lanes per SpMV row
   auto time_ms = [&](auto fn) -> double {
     std::vector<float> t;
     for (int r = 0; r <= reps; r++) {  // first run warms up
@@ -1664,14 +1703,16 @@ int rchol_amd_autotune(rchol_amd_ctx *c, int reps, int *trsv_lanes, int *spmv_la
   };
   int best_t = c->trsv_lanes, best_s = c->spmv_lanes;
   double bt = 1e30, bs = 1e30;
-  for (int L : cand) {
+  for (int L : cand_t) {
     c->trsv_lanes = L;
     rc = c->size_trsv_grid();
     if (rc) return rc;
     double ms = time_ms([&]() { (void)c->precond(c->b.p, c->tmp64a.p); });
     if (ms < bt) { bt = ms; best_t = L; }
+  }
+  for (int L : cand_s) {
     c->spmv_lanes = L;
-    ms = time_ms([&]() { launch_spmv_res(c, c->b.p, nullptr, c->q.p); });
+    double ms = time_ms([&]() { launch_spmv_res(c, c->b.p, nullptr, c->q.p); });
     if (ms < bs) { bs = ms; best_s = L; }
   }
   c->trsv_lanes = best_t;

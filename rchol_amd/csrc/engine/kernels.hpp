@@ -11,41 +11,123 @@ namespace rchol_amd {
 // ---------------------------------------------------------------------------
 // Sparse triangular sweep, pull form, persistent and level-scheduled.
 //
-// Rows are stored in "slot" order: sorted by dependency level, so a slot
-// range of one level holds mutually independent rows.  The host cuts each
-// level into tasks of at most 64 slots; task t belongs to wave t % nwaves
-// (static round robin, no atomics).  A row reads x[j] of its dependencies;
-// unpublished entries still hold the sentinel and are polled until the
-// producing wave stores the value (agent-scope atomic store).  Deadlock
-// freedom: every launched wave is resident (the grid is sized from the
-// occupancy query), each wave walks its tasks in increasing order and a task
-// only depends on tasks of lower levels, so the smallest unfinished task
-// always has all inputs and is being worked on.
+// Rows are stored in "slot" order: sorted by dependency level (and inside a
+// level by decreasing length), so a slot range of one level holds mutually
+// independent rows.  The host cuts each level into tasks of at most 64
+// slots and gives every task a lane width L = 2^lg (lg in [0, 6]) from the
+// length of its longest row; task t belongs to wave t % nwaves (static round
+// robin, no atomics).  A row reads x[j] of its dependencies; unpublished
+// entries still hold the sentinel and are polled until the producing wave
+// stores the value (agent-scope atomic store).  Deadlock freedom: every
+// launched wave is resident (the grid is sized from the occupancy query),
+// each wave walks its tasks in increasing order and a task only depends on
+// tasks of lower levels, so the smallest unfinished task always has all
+// inputs and is being worked on.
 //
 //   x[row] = (rhs[row] - sum_e val[e] * x[col[e]]) / diag[slot]
+//
+// A row group of L lanes takes its row in rounds of kGather * L entries:
+// every lane loads its kGather indices and values, then issues all
+// kGather gathers before looking at any of them, so a round costs one
+// index latency and one gather latency whatever its length.  The lane
+// order of the products (entry e0 + sub, + L, + 2L, ...) and the group sum
+// do not depend on kGather.
 //
 // FUSE == 1: also store the sentinel into aux[row] (prepares the output of
 //            the following sweep, which must start from sentinels).
 // FUSE == 2: accumulate dotv[row] * x[row] and reduce it on the grid into
 //            sc->rz; the last block forms beta (pcg.cpp:93-95).
 // ---------------------------------------------------------------------------
+#ifndef RCHOL_AMD_GATHER
+#define RCHOL_AMD_GATHER 4
+#endif
+constexpr int kGather = RCHOL_AMD_GATHER;  // gathers in flight per lane and round
+
+// Sum of val[e] * x[ci[e]] over the entries e0 + sub + k L < e1 of one row,
+// waiting for unpublished x.  Wave-converged call (every lane of the wave,
+// e0 == e1 for idle groups).
+template <typename VT, typename XT, int L, bool WAIT>
+__device__ __forceinline__ XT row_dot(int e0, int e1, int sub, const int *__restrict__ ci,
+                                      const VT *__restrict__ cv, const XT *x, int *err,
+                                      int coherent) {
+  XT acc = 0;
+  for (int c = e0 + sub; c < e1; c += kGather * L) {
+    int j[kGather];
+    VT v[kGather];
+    XT xv[kGather];
+#pragma unroll
+    for (int u = 0; u < kGather; u++) {
+      const int e = c + u * L;
+      j[u] = e < e1 ? ci[e] : -1;
+      v[u] = e < e1 ? cv[e] : VT(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kGather; u++) {
+      if (j[u] < 0)
+        xv[u] = XT(0);
+      else if (WAIT && coherent)
+        xv[u] = Sent<XT>::load_agent(x + j[u]);
+      else
+        xv[u] = x[j[u]];
+    }
+    if (WAIT) {
+      bool pend = false;
+#pragma unroll
+      for (int u = 0; u < kGather; u++) pend |= Sent<XT>::is(xv[u]);
+      if (__ballot(pend)) await_batch<XT, kGather>(x, j, xv, err);
+    }
+#pragma unroll
+    for (int u = 0; u < kGather; u++) acc += (XT)v[u] * xv[u];
+  }
+  return acc;
+}
+
+// The rows of one task (cnt <= 64 slots, metadata already in lane k of the
+// m_* registers for slot k) with row groups of L lanes.
 template <typename VT, typename XT, typename BT, int L, int FUSE>
-__global__ __launch_bounds__(256) void k_sptrsv(
-    int ntasks, const int *__restrict__ task_ptr, const int *__restrict__ slot_row,
-    const int *__restrict__ rp, const int *__restrict__ ci, const VT *__restrict__ cv,
-    const VT *__restrict__ diag, const BT *__restrict__ rhs, XT *x, XT *__restrict__ aux,
-    const double *__restrict__ dotv, double *__restrict__ partials, DevScalars *sc,
-    const int *stop, int defer, int coherent) {
-  if (*stop) return;
+__device__ __forceinline__ void sweep_task(int cnt, int m_row, int m_e0, int m_e1, VT m_dg,
+                                           const int *__restrict__ ci, const VT *__restrict__ cv,
+                                           const BT *__restrict__ rhs, XT *x,
+                                           XT *__restrict__ aux, const double *__restrict__ dotv,
+                                           int *err, int coherent, double &dacc) {
   constexpr int R = 64 / L;
   const int lane = threadIdx.x & 63;
   const int grp = lane / L, sub = lane % L;
+  for (int base = 0; base < cnt; base += R) {
+    const int k = base + grp;  // < 64: base <= cnt - 1 - ((cnt - 1) % R)
+    const int row = __shfl(m_row, k, 64);
+    int e0 = __shfl(m_e0, k, 64), e1 = __shfl(m_e1, k, 64);
+    const VT dg = __shfl(m_dg, k, 64);
+    const bool live = k < cnt;
+    if (!live) e1 = e0;
+    const BT b = (live && sub == 0) ? rhs[row] : (BT)0;
+    XT acc = row_dot<VT, XT, L, true>(e0, e1, sub, ci, cv, x, err, coherent);
+    acc = group_sum<L>(acc);
+    if (live && sub == 0) {
+      const XT v = ((XT)b - acc) / (XT)dg;
+      Sent<XT>::store_agent(x + row, v);
+      if (FUSE == 1) aux[row] = Sent<XT>::value();
+      if (FUSE == 2) dacc += dotv[row] * (double)v;
+    }
+  }
+}
+
+template <typename VT, typename XT, typename BT, int FUSE>
+__global__ __launch_bounds__(256) void k_sptrsv(
+    int ntasks, const int *__restrict__ task_ptr, const int *__restrict__ task_lg,
+    const int *__restrict__ slot_row, const int *__restrict__ rp, const int *__restrict__ ci,
+    const VT *__restrict__ cv, const VT *__restrict__ diag, const BT *__restrict__ rhs, XT *x,
+    XT *__restrict__ aux, const double *__restrict__ dotv, double *__restrict__ partials,
+    DevScalars *sc, const int *stop, int defer, int coherent) {
+  if (*stop) return;
+  const int lane = threadIdx.x & 63;
   const int nwaves = gridDim.x * (blockDim.x >> 6);
   const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   int *err = &sc->err;
   double dacc = 0.0;
   for (int t = wave; t < ntasks; t += nwaves) {
     const int s0 = task_ptr[t], s1 = task_ptr[t + 1];
+    const int lg = task_lg[t];
     const int cnt = s1 - s0;  // <= 64 slots, one coalesced load of their metadata
     int m_row = 0, m_e0 = 0, m_e1 = 0;
     VT m_dg = 1;
@@ -55,27 +137,19 @@ __global__ __launch_bounds__(256) void k_sptrsv(
       m_e1 = rp[s0 + lane + 1];
       m_dg = diag[s0 + lane];
     }
-    for (int base = 0; base < cnt; base += R) {
-      const int k = base + grp;
-      const int row = __shfl(m_row, k, 64);
-      const int e0 = __shfl(m_e0, k, 64), e1 = __shfl(m_e1, k, 64);
-      const VT dg = __shfl(m_dg, k, 64);
-      if (k >= cnt) continue;
-      const BT b = (sub == 0) ? rhs[row] : (BT)0;
-      XT acc = 0;
-      for (int e = e0 + sub; e < e1; e += L) {
-        const int j = ci[e];
-        const XT xj = wait_ready<XT>(x + j, err, coherent);
-        acc += (XT)cv[e] * xj;
-      }
-      acc = group_sum<L>(acc);
-      if (sub == 0) {
-        const XT v = ((XT)b - acc) / (XT)dg;
-        Sent<XT>::store_agent(x + row, v);
-        if (FUSE == 1) aux[row] = Sent<XT>::value();
-        if (FUSE == 2) dacc += dotv[row] * (double)v;
-      }
+#define RCHOL_SWEEP_TASK(LL)                                                                  \
+  sweep_task<VT, XT, BT, LL, FUSE>(cnt, m_row, m_e0, m_e1, m_dg, ci, cv, rhs, x, aux, dotv, err, \
+                                   coherent, dacc)
+    switch (lg) {
+      case 0: RCHOL_SWEEP_TASK(1); break;
+      case 1: RCHOL_SWEEP_TASK(2); break;
+      case 2: RCHOL_SWEEP_TASK(4); break;
+      case 3: RCHOL_SWEEP_TASK(8); break;
+      case 4: RCHOL_SWEEP_TASK(16); break;
+      case 5: RCHOL_SWEEP_TASK(32); break;
+      default: RCHOL_SWEEP_TASK(64); break;
     }
+#undef RCHOL_SWEEP_TASK
   }
   if (FUSE == 2) {
     double v[1] = {dacc}, tot[1];
@@ -93,43 +167,71 @@ __global__ __launch_bounds__(256) void k_sptrsv(
   }
 }
 
+// Rows [s0, s1) of one level in a 1024-thread workgroup, row groups of L
+// lanes; plain loads (every input is complete).
+template <typename VT, typename XT, typename BT, int L, int FUSE>
+__device__ __forceinline__ void serial_level(int s0, int s1, const int *__restrict__ slot_row,
+                                             const int *__restrict__ rp,
+                                             const int *__restrict__ ci,
+                                             const VT *__restrict__ cv,
+                                             const VT *__restrict__ diag,
+                                             const BT *__restrict__ rhs, XT *x,
+                                             XT *__restrict__ aux,
+                                             const double *__restrict__ dotv, double &dacc) {
+  const int G = blockDim.x / L;
+  const int g = threadIdx.x / L, sub = threadIdx.x % L;
+  for (int sb = s0; sb < s1; sb += G) {
+    const int s = sb + g;
+    const bool live = s < s1;
+    int row = 0, e0 = 0, e1 = 0;
+    if (live) {
+      row = slot_row[s];
+      e0 = rp[s];
+      e1 = rp[s + 1];
+    }
+    XT acc = row_dot<VT, XT, L, false>(e0, e1, sub, ci, cv, x, nullptr, 0);
+    acc = group_sum<L>(acc);
+    if (live && sub == 0) {
+      const XT v = ((XT)rhs[row] - acc) / (XT)diag[s];
+      x[row] = v;
+      if (FUSE == 1) aux[row] = Sent<XT>::value();
+      if (FUSE == 2) dacc += dotv[row] * (double)v;
+    }
+  }
+}
+
 // Narrow levels in one workgroup: levels [lev0, lev1) of a sweep, one after
 // the other, separated by workgroup barriers.  It runs as its own launch
 // before (head) or after (tail) the persistent kernel, so every dependency
 // outside the range is complete at launch and every one inside it was
 // written by this workgroup before a barrier: plain loads, no polling, no
-// cross-CU hop per level.
+// cross-CU hop per level.  level_lg[l] is the lane width of level l.
 // FUSE == 1: sentinel into aux.  FUSE == 2: r.z partial of these rows;
 // fin == 0 adds it to sc->rz_acc (a later phase finalises), fin == 1 adds
 // sc->rz_acc and finalises rz / beta (pcg.cpp:93-95).
-template <typename VT, typename XT, typename BT, int L, int FUSE>
+template <typename VT, typename XT, typename BT, int FUSE>
 __global__ __launch_bounds__(1024) void k_sptrsv_serial(
-    int lev0, int lev1, const int *__restrict__ level_ptr, const int *__restrict__ slot_row,
-    const int *__restrict__ rp, const int *__restrict__ ci, const VT *__restrict__ cv,
-    const VT *__restrict__ diag, const BT *__restrict__ rhs, XT *x, XT *__restrict__ aux,
-    const double *__restrict__ dotv, DevScalars *sc, const int *stop, int fin) {
+    int lev0, int lev1, const int *__restrict__ level_ptr, const int *__restrict__ level_lg,
+    const int *__restrict__ slot_row, const int *__restrict__ rp, const int *__restrict__ ci,
+    const VT *__restrict__ cv, const VT *__restrict__ diag, const BT *__restrict__ rhs, XT *x,
+    XT *__restrict__ aux, const double *__restrict__ dotv, DevScalars *sc, const int *stop,
+    int fin) {
   if (*stop) return;
-  const int G = blockDim.x / L;
-  const int g = threadIdx.x / L, sub = threadIdx.x % L;
   double dacc = 0.0;
   for (int l = lev0; l < lev1; l++) {
     const int s0 = level_ptr[l], s1 = level_ptr[l + 1];
-    for (int sb = s0; sb < s1; sb += G) {
-      const int s = sb + g;
-      XT acc = 0;
-      int row = 0;
-      if (s < s1) {
-        row = slot_row[s];
-        for (int e = rp[s] + sub; e < rp[s + 1]; e += L) acc += (XT)cv[e] * x[ci[e]];
-      }
-      acc = group_sum<L>(acc);
-      if (s < s1 && sub == 0) {
-        const XT v = ((XT)rhs[row] - acc) / (XT)diag[s];
-        x[row] = v;
-        if (FUSE == 1) aux[row] = Sent<XT>::value();
-        if (FUSE == 2) dacc += dotv[row] * (double)v;
-      }
+#define RCHOL_SERIAL_LEVEL(LL)                                                                \
+  serial_level<VT, XT, BT, LL, FUSE>(s0, s1, slot_row, rp, ci, cv, diag, rhs, x, aux, dotv, dacc)
+    switch (level_lg[l]) {
+      case 0: RCHOL_SERIAL_LEVEL(1); break;
+      case 1: RCHOL_SERIAL_LEVEL(2); break;
+      case 2: RCHOL_SERIAL_LEVEL(4); break;
+      case 3: RCHOL_SERIAL_LEVEL(8); break;
+      case 4: RCHOL_SERIAL_LEVEL(16); break;
+      case 5: RCHOL_SERIAL_LEVEL(32); break;
+      default: RCHOL_SERIAL_LEVEL(64); break;
     }
+#undef RCHOL_SERIAL_LEVEL
     __syncthreads();
   }
   if (FUSE == 2) {

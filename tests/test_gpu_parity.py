@@ -253,7 +253,7 @@ def test_autotune_keeps_results():
     print("case autotune", flush=True)
     with rchol_amd.Solver(A, Gs) as s:
         t = s.autotune(reps=2)
-        assert t["trsv_lanes"] in (4, 8, 16) and t["spmv_lanes"] in (4, 8, 16)
+        assert t["trsv_lanes"] in (1, 2, 4, 8) and t["spmv_lanes"] in (4, 8, 16)
         z = s.precond(d["b"])
         x, relres, itr = s.pcg(d["b"], 1e-10, 5000)
     assert rel(z, d["z"]) <= PRECOND_RTOL
