@@ -58,8 +58,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ref", action="store_true", help="skip the 1e-14 FP64 reference solve")
     ap.add_argument("--graph-iters", type=int, default=16)
-    ap.add_argument("--trsv-lanes", type=int, default=8)
-    ap.add_argument("--spmv-lanes", type=int, default=8)
+    ap.add_argument("--trsv-lanes", type=int, default=1,
+                    help="minimum lanes per sweep row (tasks widen to their longest row)")
+    ap.add_argument("--spmv-lanes", type=int, default=0,
+                    help="lanes per SpMV row; 0 = stream form")
     ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4a", "c4b", "c5"],
                     help="c3 (default) is the headline; the others print their own line")
     ap.add_argument("--nrhs", type=int, default=32)

@@ -40,7 +40,8 @@ typedef struct rchol_amd_ctx rchol_amd_ctx;
 /* Option keys for rchol_amd_set_option. */
 #define RCHOL_AMD_OPT_GRAPH_ITERS 1   /* PCG iterations per captured hipGraph (0 = no graph) */
 #define RCHOL_AMD_OPT_RESID_REPLACE 2 /* true-residual replacement period (0 = never) */
-#define RCHOL_AMD_OPT_SPMV_LANES 3    /* lanes per row in SpMV (4, 8, 16 or 32) */
+#define RCHOL_AMD_OPT_SPMV_LANES 3    /* lanes per row in SpMV (4, 8, 16 or 32), or 0
+                                         (default): stream form, 64 rows per wave step */
 #define RCHOL_AMD_OPT_TRSV_LANES 4    /* minimum lanes per row in the triangular sweeps
                                          (power of two <= 64, default 1); a task
                                          widens to cover its longest row in one
@@ -49,6 +50,12 @@ typedef struct rchol_amd_ctx rchol_amd_ctx;
 #define RCHOL_AMD_OPT_SERIAL_WIDTH 6  /* leading/trailing sweep levels of at most this many
                                          rows run in one workgroup (0 = off) */
 #define RCHOL_AMD_OPT_COHERENT_GATHER 7 /* 1: sweeps gather x with agent-scope loads */
+#define RCHOL_AMD_OPT_TASK_WAVES 8    /* waves the sweep task cut plans for (0 = the
+                                         resident capacity): fewer makes wide-level
+                                         tasks (and stream tasks) appear earlier */
+#define RCHOL_AMD_OPT_STRICT_SYNC 9   /* 1: the PCG loop's grid reductions publish their
+                                         partials with release/acquire fences instead of
+                                         value-returning atomics (same results) */
 
 /* Counters filled by rchol_amd_get_stats. */
 typedef struct rchol_amd_stats {
@@ -182,7 +189,8 @@ int rchol_amd_edge_grad_device(rchol_amd_ctx *ctx, size_t m, const long long *d_
                                double *d_g);
 
 /* Autotuner: times the preconditioner application with a minimum of 1, 2,
- * 4 and 8 lanes per sweep row and the SpMV with 4, 8 and 16 lanes per row on
+ * 4 and 8 lanes per sweep row and the SpMV in stream form and with 4, 8 and 16
+ * lanes per row on
  * the uploaded matrices (median of reps runs) and keeps the fastest
  * (RCHOL_AMD_OPT_TRSV_LANES / _SPMV_LANES).  Outputs may be NULL.  No
  * reference counterpart. */

@@ -27,6 +27,7 @@ struct DevScalars {
   int err;          // bit 0: a triangular-solve wait hit its bound
   int zero;         // constant 0: the stop flag of a standalone sweep
   int resid_period; // true-residual replacement period (0 = off)
+  int strict;       // 1: grid reductions publish partials with release/acquire fences
   unsigned int ctr[8];  // last-block counters of the reductions (self-resetting)
 };
 
@@ -236,31 +237,57 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
   __syncthreads();
 }
 
-// Last-block reduction: every block stores its NV partial sums; the block
-// that arrives last sums all partials in block order (deterministic) and
-// returns true with the totals in tot[] (thread 0 only).  The counter resets
-// itself, so the same kernel can be replayed from a captured graph.
+// Last-block reduction: every block publishes its NV partial sums and
+// counts itself in; the block that arrives last sums all partials in block
+// order (deterministic) and returns true with the totals in tot[] (thread 0
+// only).  The counter resets itself, so the same kernel can be replayed from
+// a captured graph.
+//
+// Publication.  strict == 1 is the textbook form: plain stores, then an
+// acquire-release counter increment (an agent-scope release writes back the
+// whole L2 of the XCD, once per block).  strict == 0 (default) keeps every
+// byte of the handshake in agent-scope atomics instead: a partial is written
+// by a value-returning exchange, the counter increment is issued only once
+// every exchange has returned (an empty asm consumes the returned values),
+// and the last block reads the partials with agent-scope atomic loads.  A
+// returned read-modify-write has been performed at the point where every
+// agent-scope atomic of every XCD is ordered, so the last block finds all
+// partials there without any cache writeback.
 template <int NV>
 __device__ __forceinline__ bool grid_reduce(double (&v)[NV], double *partials, unsigned int *ctr,
-                                            double (&tot)[NV]) {
+                                            double (&tot)[NV], int strict) {
   block_sum<NV>(v);
   __shared__ int is_last;
   if (threadIdx.x == 0) {
+    unsigned int prev;
+    if (strict) {
 #pragma unroll
-    for (int i = 0; i < NV; i++) partials[(size_t)blockIdx.x * NV + i] = v[i];
-    unsigned int prev =
-        __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      for (int i = 0; i < NV; i++) partials[(size_t)blockIdx.x * NV + i] = v[i];
+      prev = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      unsigned long long got = 0;
+#pragma unroll
+      for (int i = 0; i < NV; i++)
+        got ^= __hip_atomic_exchange(
+            (unsigned long long *)(partials + (size_t)blockIdx.x * NV + i),
+            (unsigned long long)__double_as_longlong(v[i]), __ATOMIC_RELAXED,
+            __HIP_MEMORY_SCOPE_AGENT);
+      asm volatile("" ::"v"(got) : "memory");
+      prev = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     is_last = (prev == gridDim.x - 1);
   }
   __syncthreads();
   if (!is_last) return false;
-  __atomic_thread_fence(__ATOMIC_ACQUIRE);
+  if (strict) __atomic_thread_fence(__ATOMIC_ACQUIRE);
   double s[NV];
 #pragma unroll
   for (int i = 0; i < NV; i++) s[i] = 0.0;
   for (unsigned int b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
 #pragma unroll
-    for (int i = 0; i < NV; i++) s[i] += partials[(size_t)b * NV + i];
+    for (int i = 0; i < NV; i++)
+      s[i] += strict ? partials[(size_t)b * NV + i]
+                     : Sent<double>::load_agent(partials + (size_t)b * NV + i);
   }
   block_sum<NV>(s);
   if (threadIdx.x == 0) {

@@ -82,7 +82,8 @@ struct Sweep {
   // split schedule: levels [0, head_end) and [tail_begin, nlevels) run in one
   // workgroup (k_sptrsv_serial), the middle in the persistent kernel
   std::vector<int> level_ptr_h, rp_h;
-  DevBuf<int> level_ptr, level_lg, mid_task_ptr, mid_task_lg;
+  DevBuf<int> level_ptr, level_lg;
+  DevBuf<int> wave_ptr, wt_s0, wt_info;  // middle tasks in per-wave lists
   int head_end = 0, tail_begin = 0, mid_ntasks = 0;
   DevBuf<double> cv64, diag64;
   DevBuf<float> cv32, diag32;
@@ -95,27 +96,18 @@ struct HostSweep {
   int nlevels = 0;
 };
 
-// Rows sorted by level and, inside a level, by decreasing length (two
-// stable counting sorts), then level-chunked 64-slot tasks.
-void build_slots(const std::vector<int> &lev, int nlev, const std::vector<int> &len,
-                 HostSweep &hs, std::vector<int> &slot_of) {
+// Rows sorted by level (stable counting sort: inside a level the rows keep
+// their nested-dissection order, so a run of slots is a compact piece of
+// the domain), then level-chunked 64-slot tasks.
+void build_slots(const std::vector<int> &lev, int nlev, HostSweep &hs, std::vector<int> &slot_of) {
   const int n = (int)lev.size();
-  int maxlen = 0;
-  for (int i = 0; i < n; i++) maxlen = std::max(maxlen, len[i]);
-  std::vector<int> by_len(n);
-  {
-    std::vector<int> cl(maxlen + 2, 0);
-    for (int i = 0; i < n; i++) cl[maxlen - len[i] + 1]++;
-    for (int k = 0; k <= maxlen; k++) cl[k + 1] += cl[k];
-    for (int i = 0; i < n; i++) by_len[cl[maxlen - len[i]]++] = i;
-  }
   std::vector<int> cnt(nlev + 1, 0);
   for (int i = 0; i < n; i++) cnt[lev[i] + 1]++;
   for (int l = 0; l < nlev; l++) cnt[l + 1] += cnt[l];
   hs.slot_row.assign(n, 0);
   slot_of.assign(n, 0);
   std::vector<int> cur(cnt.begin(), cnt.end() - 1);
-  for (int i : by_len) {
+  for (int i = 0; i < n; i++) {
     int s = cur[lev[i]]++;
     hs.slot_row[s] = i;
     slot_of[i] = s;
@@ -148,49 +140,120 @@ int lane_lg(int len, int lmin_lg) {
   return lg;
 }
 
-// Tasks of the middle levels [h, t).  Slots of a level are sorted by
-// decreasing length, so a task's lane width is set by its first row.  A
-// task holds at most 64 slots: as many row passes (64/L rows each) as it
-// takes for the level to need about one task per wave, so narrow levels
-// spread over the waves and wide ones amortise the per-task overhead.
+// Tasks of the middle levels [h, t), in level order and, inside a level,
+// in slot (nested-dissection) order.  A level is wide when one-pass tasks
+// of 16 rows would outnumber the waves: its slots go to stream tasks of up
+// to 64 rows and 64 * kStream entries (one round of gathers).  A narrow
+// level is cut into one-pass lane tasks whose width L covers the longest of
+// its 64/L rows in one round of kGather gathers per lane, so it spreads
+// over as many waves as it can use.
+struct TaskCut {
+  std::vector<int> s0, info, level;  // info = cnt | lg << kTaskInfoShift
+};
+
 void cut_tasks(const std::vector<int> &lp, const std::vector<int> &rp, int h, int t, int lmin_lg,
-               long long nwaves, std::vector<int> &ptr, std::vector<int> &lgs) {
-  ptr.clear();
-  lgs.clear();
+               long long nwaves, TaskCut &tc) {
+  tc.s0.clear();
+  tc.info.clear();
+  tc.level.clear();
+  auto push = [&](int s, int cnt, int lg, int l) {
+    tc.s0.push_back(s);
+    tc.info.push_back(cnt | (lg << kTaskInfoShift));
+    tc.level.push_back(l);
+  };
   for (int l = h; l < t; l++) {
     const int s_end = lp[l + 1];
     const long long rows = s_end - lp[l];
+    const bool wide = rows > 16 * nwaves;
     for (int s = lp[l]; s < s_end;) {
-      const int lg = lane_lg(rp[s + 1] - rp[s], lmin_lg);
-      const int R = 64 >> lg;
-      const long long one_pass = (rows + R - 1) / R;
-      const int passes =
-          (int)std::min<long long>(64 / R, std::max<long long>(1, (one_pass + nwaves - 1) / nwaves));
-      ptr.push_back(s);
-      lgs.push_back(lg);
-      s += std::min(s_end - s, passes * R);
+      if (wide) {
+        int k = 1;  // at least one row, however long (the kernel loops rounds)
+        while (s + k < s_end && k < 64 && rp[s + k + 1] - rp[s] <= 64 * kStream) k++;
+        push(s, k, kTaskStream, l);
+        s += k;
+        continue;
+      }
+      int lg = lane_lg(rp[s + 1] - rp[s], lmin_lg);
+      for (;;) {  // widen until the longest of the 64/L rows fits one round
+        const int take = std::min(s_end - s, 64 >> lg);
+        int need = lg;
+        for (int k = 0; k < take; k++)
+          need = std::max(need, lane_lg(rp[s + k + 1] - rp[s + k], lmin_lg));
+        if (need == lg) break;
+        lg = need;
+      }
+      const int take = std::min(s_end - s, 64 >> lg);
+      push(s, take, lg, l);
+      s += take;
     }
   }
-  ptr.push_back(t > h ? lp[t] : lp[h]);
 }
 
-// Upload a sweep's split schedule: head/tail bounds, per-level lane widths
-// (for the one-workgroup phases) and the middle's task list.
-hipError_t split_levels(Sweep &sw, int width, int lmin_lg, long long nwaves) {
+// Static task -> wave assignment.  Block b runs on XCD b % 8 (round-robin
+// dispatch), so a level's tasks are split into 8 contiguous runs, run x
+// going to the waves of XCD x: neighbouring rows, and the rows of the same
+// piece of the domain in consecutive levels, share one L2.  Inside an XCD
+// the waves take tasks round robin with a cursor that carries across
+// levels.  Each wave's list is in level order, which is all the
+// deadlock-freedom argument of k_sptrsv needs.
+void assign_waves(const TaskCut &tc, int nblocks, std::vector<int> &wave_ptr,
+                  std::vector<int> &ws0, std::vector<int> &winfo) {
+  const int W = nblocks * 4;
+  const int X = nblocks >= 8 ? 8 : 1;
+  std::vector<std::vector<int>> xw(X);  // waves of each XCD
+  for (int b = 0; b < nblocks; b++)
+    for (int q = 0; q < 4; q++) xw[b % X].push_back(4 * b + q);
+  std::vector<size_t> cursor(X, 0);
+  std::vector<std::vector<int>> lists(W);
+  const size_t T = tc.s0.size();
+  for (size_t i = 0; i < T;) {
+    size_t j = i;
+    while (j < T && tc.level[j] == tc.level[i]) j++;
+    const size_t tl = j - i;
+    for (size_t k = 0; k < tl; k++) {
+      const int x = (int)((k * X) / tl);
+      const int w = xw[x][cursor[x]++ % xw[x].size()];
+      lists[w].push_back((int)(i + k));
+    }
+    i = j;
+  }
+  wave_ptr.assign(W + 1, 0);
+  ws0.clear();
+  winfo.clear();
+  for (int w = 0; w < W; w++) {
+    for (int k : lists[w]) {
+      ws0.push_back(tc.s0[k]);
+      winfo.push_back(tc.info[k]);
+    }
+    wave_ptr[w + 1] = (int)ws0.size();
+  }
+  if (ws0.empty()) {
+    ws0.push_back(0);
+    winfo.push_back(0);
+  }
+}
+
+// Upload a sweep's split schedule: head/tail bounds [0, h) / [t, nlevels),
+// per-level lane widths (for the one-workgroup phases) and the middle's
+// tasks listed per wave of a grid of nblocks persistent blocks.
+hipError_t upload_schedule(Sweep &sw, int h, int t, int lmin_lg, const TaskCut &tc, int nblocks) {
   const std::vector<int> &lp = sw.level_ptr_h;
-  int h, t;
-  compute_split(lp, sw.nlevels, width, h, t);
-  std::vector<int> tasks, lgs, llg(std::max(sw.nlevels, 1), lmin_lg);
-  for (int l = 0; l < sw.nlevels; l++)
-    if (lp[l + 1] > lp[l]) llg[l] = lane_lg(sw.rp_h[lp[l] + 1] - sw.rp_h[lp[l]], lmin_lg);
-  cut_tasks(lp, sw.rp_h, h, t, lmin_lg, nwaves, tasks, lgs);
+  std::vector<int> llg(std::max(sw.nlevels, 1), lmin_lg);
+  for (int l = 0; l < sw.nlevels; l++) {
+    int lg = lmin_lg;
+    for (int k = lp[l]; k < lp[l + 1]; k++)
+      lg = std::max(lg, lane_lg(sw.rp_h[k + 1] - sw.rp_h[k], lmin_lg));
+    llg[l] = lg;
+  }
   sw.head_end = h;
   sw.tail_begin = t;
-  sw.mid_ntasks = (int)tasks.size() - 1;
-  if (lgs.empty()) lgs.push_back(lmin_lg);
+  sw.mid_ntasks = (int)tc.s0.size();
+  std::vector<int> wp, ws0, winfo;
+  assign_waves(tc, nblocks, wp, ws0, winfo);
   hipError_t e = sw.level_lg.upload(llg.data(), llg.size());
-  if (e == hipSuccess) e = sw.mid_task_lg.upload(lgs.data(), lgs.size());
-  if (e == hipSuccess) e = sw.mid_task_ptr.upload(tasks.data(), tasks.size());
+  if (e == hipSuccess) e = sw.wave_ptr.upload(wp.data(), wp.size());
+  if (e == hipSuccess) e = sw.wt_s0.upload(ws0.data(), ws0.size());
+  if (e == hipSuccess) e = sw.wt_info.upload(winfo.data(), winfo.size());
   return e;
 }
 
@@ -227,12 +290,14 @@ struct rchol_amd_ctx {
   // options
   int graph_iters = 16;
   int resid_period = 0;
-  int spmv_lanes = 8;
+  int spmv_lanes = 0;  // lanes per SpMV row; 0 = stream form (64 rows per wave step)
   int trsv_lanes = 1;  // minimum lanes per sweep row (tasks widen to fit)
   int profile = 0;
   int profile_no_graph = 0;
   int serial_width = 0;  // narrow-level threshold of the split sweeps (0 = off)
   int coherent_gather = 0;  // sweeps read x with agent-scope loads from the start
+  int task_waves = 0;       // waves the task cut plans for (0 = resident capacity)
+  int strict_sync = 0;      // grid reductions with release/acquire fences
   std::vector<hipEvent_t> pev;  // 5 per captured iteration when profiling
   // captured iteration graph
   hipGraph_t graph = nullptr;
@@ -304,9 +369,11 @@ struct rchol_amd_ctx {
     return (int)std::max<size_t>(1, std::min<size_t>(g, (size_t)ncu * 8));
   }
   int spmv_grid() const {
-    size_t rows_per_block = (kBlock / 64) * (64 / spmv_lanes);
+    size_t rows_per_block = (kBlock / 64) * (spmv_lanes ? 64 / spmv_lanes : 64);
     size_t g = (n + rows_per_block - 1) / rows_per_block;
-    return (int)std::max<size_t>(1, std::min<size_t>(g, (size_t)ncu * 8));
+    g = std::max<size_t>(1, std::min<size_t>(g, (size_t)ncu * 8));
+    if (g > 8) g = (g + 7) / 8 * 8;  // whole XCD rounds for the row walk of the stream form
+    return (int)g;
   }
   int ensure_work();
   int size_trsv_grid();
@@ -348,7 +415,7 @@ void launch_one_sweep(rchol_amd_ctx *c, const Sweep &sw, const VT *cv, const VT 
                        sw.ci.p, cv, dg, rhs, x, aux, dotv, c->sc, stop, 0);
   if (sw.mid_ntasks > 0 || (FUSE == 2 && !tail))
     hipLaunchKernelGGL((k_sptrsv<VT, XT, BT, FUSE>), dim3(c->trsv_grid), dim3(kBlock), 0,
-                       c->stream, sw.mid_ntasks, sw.mid_task_ptr.p, sw.mid_task_lg.p,
+                       c->stream, sw.wave_ptr.p, sw.wt_s0.p, sw.wt_info.p,
                        sw.slot_row.p, sw.rp.p, sw.ci.p, cv, dg, rhs, x, aux, dotv, c->partials.p,
                        c->sc, stop, tail ? 1 : 0, c->coherent_gather);
   if (tail)
@@ -388,6 +455,11 @@ void launch_spmv_p_t(rchol_amd_ctx *c, const ZT *z, const double *p_old, double 
 template <typename ZT>
 void launch_spmv_p(rchol_amd_ctx *c, const ZT *z, const double *p_old, double *p_new) {
   switch (c->spmv_lanes) {
+    case 0:
+      hipLaunchKernelGGL((k_spmv_p_s<ZT>), dim3(c->spmv_grid()), dim3(kBlock), 0, c->stream,
+                         (int)c->n, c->arp.p, c->aci.p, c->av.p, z, p_old, p_new, c->q.p, c->r.p,
+                         c->partials.p, c->sc);
+      break;
     case 4: launch_spmv_p_t<ZT, 4>(c, z, p_old, p_new); break;
     case 16: launch_spmv_p_t<ZT, 16>(c, z, p_old, p_new); break;
     case 32: launch_spmv_p_t<ZT, 32>(c, z, p_old, p_new); break;
@@ -410,6 +482,11 @@ void launch_resid_replace_t(rchol_amd_ctx *c) {
 
 void launch_resid_replace(rchol_amd_ctx *c) {
   switch (c->spmv_lanes) {
+    case 0:
+      hipLaunchKernelGGL(k_resid_replace_s, dim3(c->spmv_grid()), dim3(kBlock), 0, c->stream,
+                         (int)c->n, c->arp.p, c->aci.p, c->av.p, (const double *)c->x.p,
+                         (const double *)c->b.p, c->r.p, c->partials.p, c->sc);
+      break;
     case 4: launch_resid_replace_t<4>(c); break;
     case 16: launch_resid_replace_t<16>(c); break;
     case 32: launch_resid_replace_t<32>(c); break;
@@ -419,6 +496,10 @@ void launch_resid_replace(rchol_amd_ctx *c) {
 
 void launch_spmv_res(rchol_amd_ctx *c, const double *x, const double *b, double *y) {
   switch (c->spmv_lanes) {
+    case 0:
+      hipLaunchKernelGGL(k_spmv_res_s, dim3(c->spmv_grid()), dim3(kBlock), 0, c->stream, (int)c->n,
+                         c->arp.p, c->aci.p, c->av.p, x, b, y, c->partials.p, c->sc);
+      break;
     case 4: launch_spmv_res_t<4>(c, x, b, y); break;
     case 16: launch_spmv_res_t<16>(c, x, b, y); break;
     case 32: launch_spmv_res_t<32>(c, x, b, y); break;
@@ -427,6 +508,13 @@ void launch_spmv_res(rchol_amd_ctx *c, const double *x, const double *b, double 
 }
 
 }  // namespace
+
+// Workgroups per CU a persistent grid may count on being co-resident: the
+// occupancy query less one.  The query packs register files exactly, while
+// the allocation granules of the hardware (SGPR blocks in particular) can
+// leave one workgroup per CU short of it; a persistent sweep whose grid is
+// not fully resident waits on tasks that never start.
+static int resident_blocks(int query) { return std::max(1, std::min(query, 8) - 1); }
 
 // Persistent sweep grid: every block must be resident at once (the
 // deadlock-freedom argument of k_sptrsv).  The occupancy is the minimum
@@ -450,14 +538,22 @@ int rchol_amd_ctx::size_trsv_grid() {
   int per_cu = gprec == RCHOL_AMD_FP32 ? trsv_occupancy<float, float>()
                                        : trsv_occupancy<double, double>();
   if (per_cu <= 0) return fail(RCHOL_AMD_EHIP, "occupancy query for the sweep kernels failed");
-  per_cu = std::min(per_cu, 8);
-  const long long cap_waves = (long long)per_cu * ncu * (kBlock / 64);
+  per_cu = resident_blocks(per_cu);
+  const int cap_blocks = per_cu * ncu;
+  const long long plan_waves = task_waves > 0 ? task_waves : (long long)cap_blocks * (kBlock / 64);
   int lmin_lg = 0;
   while ((1 << lmin_lg) < trsv_lanes) lmin_lg++;
-  HIP_TRY(split_levels(fwd, serial_width, lmin_lg, cap_waves));
-  HIP_TRY(split_levels(bwd, serial_width, lmin_lg, cap_waves));
-  const int max_tasks = std::max(fwd.mid_ntasks, bwd.mid_ntasks);
-  trsv_grid = std::max(1, std::min(per_cu * ncu, (max_tasks + 3) / 4));
+  int h[2], t[2];
+  TaskCut tc[2];
+  Sweep *sw[2] = {&fwd, &bwd};
+  size_t max_tasks = 0;
+  for (int k = 0; k < 2; k++) {
+    compute_split(sw[k]->level_ptr_h, sw[k]->nlevels, serial_width, h[k], t[k]);
+    cut_tasks(sw[k]->level_ptr_h, sw[k]->rp_h, h[k], t[k], lmin_lg, plan_waves, tc[k]);
+    max_tasks = std::max(max_tasks, tc[k].s0.size());
+  }
+  trsv_grid = (int)std::max<size_t>(1, std::min<size_t>(cap_blocks, (max_tasks + 3) / 4));
+  for (int k = 0; k < 2; k++) HIP_TRY(upload_schedule(*sw[k], h[k], t[k], lmin_lg, tc[k], trsv_grid));
   stats.trsv_waves = trsv_grid * (kBlock / 64);
   stats.fwd_tasks = fwd.mid_ntasks;
   stats.bwd_tasks = bwd.mid_ntasks;
@@ -523,6 +619,8 @@ int rchol_amd_ctx::pcg(const double *d_b, double tol, int maxit, double *d_x_out
   hsc->maxit = maxit;
   hsc->zero = 0;
   hsc->resid_period = resid_period;
+  hsc->strict = strict_sync;
+  HIP_TRY(hipMemcpyAsync(&sc->strict, &hsc->strict, sizeof(int), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(&sc->tol, &hsc->tol, sizeof(double), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(&sc->maxit, &hsc->maxit, sizeof(int), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(&sc->zero, &hsc->zero, sizeof(int), hipMemcpyHostToDevice, stream));
@@ -691,7 +789,7 @@ int rchol_amd_ctx::ensure_batch(int K) {
     default: per_cu = gprec == RCHOL_AMD_FP32 ? trsv_occupancy_mr<float, float, 32>() : trsv_occupancy_mr<double, double, 32>(); break;
   }
   if (per_cu <= 0) return fail(RCHOL_AMD_EHIP, "occupancy query for the batched sweeps failed");
-  per_cu = std::min(per_cu, 8);
+  per_cu = resident_blocks(per_cu);
   const int max_tasks = std::max(fwd.ntasks, bwd.ntasks);
   batch.grid = std::max(1, std::min(per_cu * ncu, (max_tasks + 3) / 4));
   batch.K = K;
@@ -1216,7 +1314,7 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
   }
   std::vector<int> slot_of;
   // forward sweep: CSR of lower G in slot order
-  build_slots(levf, nlf, offf, hf, slot_of);
+  build_slots(levf, nlf, hf, slot_of);
   hf.rp.assign(n + 1, 0);
   for (size_t s = 0; s < n; s++) hf.rp[s + 1] = hf.rp[s] + offf[hf.slot_row[s]];
   const size_t noff = nnz - n;
@@ -1238,11 +1336,7 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
     for (size_t s = 0; s < n; s++) hf.diag[s] = diag[hf.slot_row[s]];
   }
   // backward sweep: the stored rows (CSR of G^T) in slot order
-  {
-    std::vector<int> offb(n);
-    for (size_t j = 0; j < n; j++) offb[j] = (int)(colPtr[j + 1] - colPtr[j] - 1);
-    build_slots(levb, nlb, offb, hb, slot_of);
-  }
+  build_slots(levb, nlb, hb, slot_of);
   hb.rp.assign(n + 1, 0);
   for (size_t s = 0; s < n; s++) {
     int row = hb.slot_row[s];
@@ -1342,7 +1436,8 @@ int rchol_amd_set_option(rchol_amd_ctx *c, int key, double value) {
       c->resid_period = v;
       break;
     case RCHOL_AMD_OPT_SPMV_LANES:
-      if (v != 4 && v != 8 && v != 16 && v != 32) return fail(RCHOL_AMD_EINVAL, "lanes must be 4/8/16/32");
+      if (v != 0 && v != 4 && v != 8 && v != 16 && v != 32)
+        return fail(RCHOL_AMD_EINVAL, "SpMV lanes must be 0 (stream), 4, 8, 16 or 32");
       c->spmv_lanes = v;
       break;
     case RCHOL_AMD_OPT_TRSV_LANES:
@@ -1356,6 +1451,13 @@ int rchol_amd_set_option(rchol_amd_ctx *c, int key, double value) {
     case RCHOL_AMD_OPT_COHERENT_GATHER:
       c->coherent_gather = v ? 1 : 0;
       break;
+    case RCHOL_AMD_OPT_STRICT_SYNC:
+      c->strict_sync = v ? 1 : 0;
+      break;
+    case RCHOL_AMD_OPT_TASK_WAVES:
+      if (v < 0) return fail(RCHOL_AMD_EINVAL, "negative task wave count");
+      c->task_waves = v;
+      break;
     case RCHOL_AMD_OPT_SERIAL_WIDTH:
       if (v < 0 || v > 1 << 20) return fail(RCHOL_AMD_EINVAL, "serial width %d out of range", v);
       c->serial_width = v;
@@ -1366,7 +1468,8 @@ int rchol_amd_set_option(rchol_amd_ctx *c, int key, double value) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   c->drop_graph();
-  if ((key == RCHOL_AMD_OPT_TRSV_LANES || key == RCHOL_AMD_OPT_SERIAL_WIDTH) && c->hasG)
+  if ((key == RCHOL_AMD_OPT_TRSV_LANES || key == RCHOL_AMD_OPT_SERIAL_WIDTH ||
+       key == RCHOL_AMD_OPT_TASK_WAVES) && c->hasG)
     return c->size_trsv_grid();
   return RCHOL_AMD_OK;
 }
@@ -1686,7 +1789,7 @@ int rchol_amd_autotune(rchol_amd_ctx *c, int reps, int *trsv_lanes, int *spmv_la
   for (size_t i = 0; i < n; i++) h[i] = 1.0 + (double)((i * 2654435761u) % 1000) * 1e-3;
   HIP_TRY(hipMemcpy(c->b.p, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
   const int cand_t[4] = {1, 2, 4, 8};  // minimum lanes per sweep row
-  const int cand_s[3] = {4, 8, 16};    // lanes per SpMV row
+  const int cand_s[4] = {0, 4, 8, 16};  // lanes per SpMV row (0 = stream form)
   auto time_ms = [&](auto fn) -> double {
     std::vector<float> t;
     for (int r = 0; r <= reps; r++) {  // first run warms up

@@ -14,15 +14,16 @@ namespace rchol_amd {
 // Rows are stored in "slot" order: sorted by dependency level (and inside a
 // level by decreasing length), so a slot range of one level holds mutually
 // independent rows.  The host cuts each level into tasks of at most 64
-// slots and gives every task a lane width L = 2^lg (lg in [0, 6]) from the
-// length of its longest row; task t belongs to wave t % nwaves (static round
-// robin, no atomics).  A row reads x[j] of its dependencies; unpublished
+// slots.  A task either has a lane width L = 2^lg (lg in [0, 6]) from the
+// length of its longest row, or -- rows of a wide level -- is a stream task
+// (sweep_stream).  The host gives every wave a static list of tasks in
+// level order (wave_ptr / wt_s0 / wt_info; no atomics).  A row reads x[j] of its dependencies; unpublished
 // entries still hold the sentinel and are polled until the producing wave
 // stores the value (agent-scope atomic store).  Deadlock freedom: every
 // launched wave is resident (the grid is sized from the occupancy query),
-// each wave walks its tasks in increasing order and a task only depends on
-// tasks of lower levels, so the smallest unfinished task always has all
-// inputs and is being worked on.
+// each wave walks its list in level order and a task only depends on rows
+// of lower levels, so among the unfinished tasks of the lowest level, the
+// one its wave is on has all inputs.
 //
 //   x[row] = (rhs[row] - sum_e val[e] * x[col[e]]) / diag[slot]
 //
@@ -112,23 +113,99 @@ __device__ __forceinline__ void sweep_task(int cnt, int m_row, int m_e0, int m_e
   }
 }
 
+// A stream task (task_lg == kTaskStream): up to 64 short rows whose
+// entries [E0, E1) are contiguous in slot order.  The wave loads the entries
+// as one coalesced block (kStream per lane and round), issues every gather
+// of the round before looking at any, parks the products in LDS and lets
+// lane k add up row k's products in entry order.  One round covers
+// 64 * kStream entries, so a task of a wide level costs one index latency
+// and one gather latency instead of one of each per row pass.
+#ifndef RCHOL_AMD_STREAM
+#define RCHOL_AMD_STREAM 8
+#endif
+constexpr int kStream = RCHOL_AMD_STREAM;  // entries per lane and round of a stream task
+constexpr int kTaskStream = 7;             // lane code of a stream task
+constexpr int kTaskInfoShift = 8;          // task info word: slot count | lane code << 8
+
+template <typename VT, typename XT, typename BT, int FUSE>
+__device__ __forceinline__ void sweep_stream(int cnt, int m_row, int m_e0, int m_e1, VT m_dg,
+                                             const int *__restrict__ ci,
+                                             const VT *__restrict__ cv,
+                                             const BT *__restrict__ rhs, XT *x,
+                                             XT *__restrict__ aux,
+                                             const double *__restrict__ dotv, int *err,
+                                             int coherent, double &dacc, XT *buf) {
+  const int lane = threadIdx.x & 63;
+  const bool live = lane < cnt;
+  const int E0 = __shfl(m_e0, 0, 64), E1 = __shfl(m_e1, cnt - 1, 64);
+  const BT b = live ? rhs[m_row] : (BT)0;
+  XT acc = 0;
+  for (int c = E0; c < E1; c += 64 * kStream) {
+    int j[kStream];
+    VT v[kStream];
+    XT xv[kStream];
+#pragma unroll
+    for (int u = 0; u < kStream; u++) {
+      const int e = c + u * 64 + lane;
+      j[u] = e < E1 ? ci[e] : -1;
+      v[u] = e < E1 ? cv[e] : VT(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kStream; u++) {
+      if (j[u] < 0)
+        xv[u] = XT(0);
+      else if (coherent)
+        xv[u] = Sent<XT>::load_agent(x + j[u]);
+      else
+        xv[u] = x[j[u]];
+    }
+    bool pend = false;
+#pragma unroll
+    for (int u = 0; u < kStream; u++) pend |= Sent<XT>::is(xv[u]);
+    if (__ballot(pend)) await_batch<XT, kStream>(x, j, xv, err);
+#pragma unroll
+    for (int u = 0; u < kStream; u++) buf[u * 64 + lane] = (XT)v[u] * xv[u];
+    // LDS requests of one wave complete in order; the fences only keep the
+    // compiler from moving the reads above the writes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (live) {
+      const int lo = max(m_e0, c) - c, hi = min(m_e1, c + 64 * kStream) - c;
+      for (int e = lo; e < hi; e++) acc += buf[e];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if (live) {
+    const XT r = ((XT)b - acc) / (XT)m_dg;
+    Sent<XT>::store_agent(x + m_row, r);
+    if (FUSE == 1) aux[m_row] = Sent<XT>::value();
+    if (FUSE == 2) dacc += dotv[m_row] * (double)r;
+  }
+}
+
 template <typename VT, typename XT, typename BT, int FUSE>
 __global__ __launch_bounds__(256) void k_sptrsv(
-    int ntasks, const int *__restrict__ task_ptr, const int *__restrict__ task_lg,
-    const int *__restrict__ slot_row, const int *__restrict__ rp, const int *__restrict__ ci,
-    const VT *__restrict__ cv, const VT *__restrict__ diag, const BT *__restrict__ rhs, XT *x,
-    XT *__restrict__ aux, const double *__restrict__ dotv, double *__restrict__ partials,
-    DevScalars *sc, const int *stop, int defer, int coherent) {
+    const int *__restrict__ wave_ptr, const int *__restrict__ wt_s0,
+    const int *__restrict__ wt_info, const int *__restrict__ slot_row,
+    const int *__restrict__ rp, const int *__restrict__ ci, const VT *__restrict__ cv,
+    const VT *__restrict__ diag, const BT *__restrict__ rhs, XT *x, XT *__restrict__ aux,
+    const double *__restrict__ dotv, double *__restrict__ partials, DevScalars *sc,
+    const int *stop, int defer, int coherent) {
   if (*stop) return;
   const int lane = threadIdx.x & 63;
-  const int nwaves = gridDim.x * (blockDim.x >> 6);
   const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   int *err = &sc->err;
   double dacc = 0.0;
-  for (int t = wave; t < ntasks; t += nwaves) {
-    const int s0 = task_ptr[t], s1 = task_ptr[t + 1];
-    const int lg = task_lg[t];
-    const int cnt = s1 - s0;  // <= 64 slots, one coalesced load of their metadata
+  __shared__ XT sbuf[4][64 * kStream];  // one stream-task product buffer per wave
+  XT *buf = sbuf[(threadIdx.x >> 6) & 3];
+  const int k1 = wave_ptr[wave + 1];
+  for (int k = wave_ptr[wave]; k < k1; k++) {
+    const int s0 = wt_s0[k], info = wt_info[k];
+    const int lg = info >> kTaskInfoShift;
+    const int cnt = info & ((1 << kTaskInfoShift) - 1);  // <= 64 slots
     int m_row = 0, m_e0 = 0, m_e1 = 0;
     VT m_dg = 1;
     if (lane < cnt) {
@@ -141,6 +218,10 @@ __global__ __launch_bounds__(256) void k_sptrsv(
   sweep_task<VT, XT, BT, LL, FUSE>(cnt, m_row, m_e0, m_e1, m_dg, ci, cv, rhs, x, aux, dotv, err, \
                                    coherent, dacc)
     switch (lg) {
+      case kTaskStream:
+        sweep_stream<VT, XT, BT, FUSE>(cnt, m_row, m_e0, m_e1, m_dg, ci, cv, rhs, x, aux, dotv, err,
+                                       coherent, dacc, buf);
+        break;
       case 0: RCHOL_SWEEP_TASK(1); break;
       case 1: RCHOL_SWEEP_TASK(2); break;
       case 2: RCHOL_SWEEP_TASK(4); break;
@@ -153,7 +234,7 @@ __global__ __launch_bounds__(256) void k_sptrsv(
   }
   if (FUSE == 2) {
     double v[1] = {dacc}, tot[1];
-    if (grid_reduce<1>(v, partials, &sc->ctr[CTR_BWD], tot)) {
+    if (grid_reduce<1>(v, partials, &sc->ctr[CTR_BWD], tot, sc->strict)) {
       if (defer) {
         sc->rz_acc += tot[0];  // a serial tail phase finalises
       } else {
@@ -324,7 +405,7 @@ __global__ __launch_bounds__(256) void k_init(int n, const double *__restrict__ 
     acc += bi * bi;
   }
   double v[1] = {acc}, tot[1];
-  if (grid_reduce<1>(v, partials, &sc->ctr[CTR_INIT], tot)) {
+  if (grid_reduce<1>(v, partials, &sc->ctr[CTR_INIT], tot, sc->strict)) {
     sc->bb = tot[0];
     sc->rr = tot[0];
     sc->k = 0;
@@ -374,7 +455,7 @@ __global__ __launch_bounds__(256) void k_spmv_p(int n, const int *__restrict__ r
     }
   }
   double v[2] = {apr, apq}, tot[2];
-  if (grid_reduce<2>(v, partials, &sc->ctr[CTR_SPMVP], tot)) {
+  if (grid_reduce<2>(v, partials, &sc->ctr[CTR_SPMVP], tot, sc->strict)) {
     sc->pr = tot[0];
     sc->pq = tot[1];
     sc->alpha = tot[0] / tot[1];
@@ -399,7 +480,7 @@ __global__ __launch_bounds__(256) void k_update(int n, const double *__restrict_
     acc += ri * ri;
   }
   double v[1] = {acc}, tot[1];
-  if (grid_reduce<1>(v, partials, &sc->ctr[CTR_UPD], tot)) {
+  if (grid_reduce<1>(v, partials, &sc->ctr[CTR_UPD], tot, sc->strict)) {
     const int k = sc->k + 1;
     sc->k = k;
     sc->rr = tot[0];
@@ -435,7 +516,7 @@ __global__ __launch_bounds__(256) void k_resid_replace(int n, const int *__restr
     }
   }
   double v[1] = {arr}, tot[1];
-  if (grid_reduce<1>(v, partials, &sc->ctr[CTR_GEN], tot)) {
+  if (grid_reduce<1>(v, partials, &sc->ctr[CTR_GEN], tot, sc->strict)) {
     sc->rr = tot[0];
     sc->done = !(sqrt(tot[0]) > sqrt(sc->bb) * sc->tol && sc->k < sc->maxit);
   }
@@ -469,7 +550,186 @@ __global__ __launch_bounds__(256) void k_spmv_res(int n, const int *__restrict__
   }
   if (bvec) {
     double v[1] = {aqq}, tot[1];
-    if (grid_reduce<1>(v, partials, &sc->ctr[CTR_RES], tot)) {
+    if (grid_reduce<1>(v, partials, &sc->ctr[CTR_RES], tot, sc->strict)) {
+      sc->qq = tot[0];
+      sc->relres = sqrt(tot[0]) / sqrt(sc->bb);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stream forms of the SpMV stages (RCHOL_AMD_OPT_SPMV_LANES = 0): a wave
+// takes 64 consecutive rows at a time, loads their entries [E0, E1) as one
+// coalesced block (kSpmvStream per lane and round), issues every gather of
+// the round at once, parks the products in LDS and lane k adds up row k in
+// entry order.  For the short rows of a stencil matrix this is one index
+// and one gather latency per 64 rows instead of one per row pass.
+// ---------------------------------------------------------------------------
+constexpr int kSpmvStream = 8;
+
+// The rows a wave walks: blocks run on XCD blockIdx.x % 8 (round-robin
+// dispatch; grids are a multiple of 8 blocks), so XCD x takes the
+// contiguous range [n x / 8, n (x + 1) / 8) and its waves step through it 64
+// rows at a time.  The neighbours gathered by one L2 are then the rows near
+// its own range instead of rows spread over the whole vector.
+struct RowWalk {
+  int hi, first, step;
+};
+__device__ __forceinline__ RowWalk xcd_rows(int n) {
+  const int X = (gridDim.x % 8 == 0) ? 8 : 1;
+  const int x = blockIdx.x % X, jb = blockIdx.x / X, nbx = gridDim.x / X;
+  const int wpb = blockDim.x >> 6;
+  const int lo = (int)((long long)n * x / X);
+  RowWalk w;
+  w.hi = (int)((long long)n * (x + 1) / X);
+  w.first = lo + (jb * wpb + (threadIdx.x >> 6)) * 64;
+  w.step = nbx * wpb * 64;
+  return w;
+}
+
+// sum_e av[e] * g(ci[e]) over the entries of row r0 + lane (lanes whose row
+// is >= n, the end of the walk, return 0).  buf: this wave's
+// 64 * kSpmvStream doubles of LDS.
+template <typename G>
+__device__ __forceinline__ double stream_rows(int r0, int n, const int *__restrict__ rp,
+                                              const int *__restrict__ ci,
+                                              const double *__restrict__ av, G g, double *buf) {
+  const int lane = threadIdx.x & 63;
+  const int row = r0 + lane;
+  const bool live = row < n;
+  const int e0 = live ? rp[row] : 0, e1 = live ? rp[row + 1] : 0;
+  const int last = min(64, n - r0) - 1;
+  const int E0 = __shfl(e0, 0, 64), E1 = __shfl(e1, last, 64);
+  double acc = 0.0;
+  for (int c = E0; c < E1; c += 64 * kSpmvStream) {
+    int j[kSpmvStream];
+    double a[kSpmvStream], pv[kSpmvStream];
+#pragma unroll
+    for (int u = 0; u < kSpmvStream; u++) {
+      const int e = c + u * 64 + lane;
+      j[u] = e < E1 ? ci[e] : -1;
+      a[u] = e < E1 ? av[e] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSpmvStream; u++) pv[u] = j[u] >= 0 ? g(j[u]) : 0.0;
+#pragma unroll
+    for (int u = 0; u < kSpmvStream; u++) buf[u * 64 + lane] = a[u] * pv[u];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (live) {
+      const int lo = max(e0, c) - c, hi = min(e1, c + 64 * kSpmvStream) - c;
+      for (int e = lo; e < hi; e++) acc += buf[e];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  return acc;
+}
+
+// k_spmv_p in stream form (same results up to the order of each row sum).
+template <typename ZT>
+__global__ __launch_bounds__(256) void k_spmv_p_s(int n, const int *__restrict__ rp,
+                                                  const int *__restrict__ ci,
+                                                  const double *__restrict__ av,
+                                                  const ZT *__restrict__ z,
+                                                  const double *__restrict__ p_old,
+                                                  double *__restrict__ p_new,
+                                                  double *__restrict__ q,
+                                                  const double *__restrict__ r, double *partials,
+                                                  DevScalars *sc) {
+  if (sc->done) return;
+  __shared__ double sbuf[4][64 * kSpmvStream];
+  double *buf = sbuf[(threadIdx.x >> 6) & 3];
+  const double beta = sc->beta;
+  const bool first = (sc->k == 0);
+  const int lane = threadIdx.x & 63;
+  const RowWalk rw = xcd_rows(n);
+  auto pval = [&](int j) -> double {
+    return first ? (double)z[j] : beta * p_old[j] + (double)z[j];
+  };
+  double apr = 0.0, apq = 0.0;
+  for (int r0 = rw.first; r0 < rw.hi; r0 += rw.step) {
+    const int row = r0 + lane;
+    double pi = 0.0, ri = 0.0;
+    if (row < rw.hi) {
+      pi = pval(row);
+      ri = r[row];
+    }
+    const double acc = stream_rows(r0, rw.hi, rp, ci, av, pval, buf);
+    if (row < rw.hi) {
+      p_new[row] = pi;
+      q[row] = acc;
+      apr += pi * ri;
+      apq += pi * acc;
+    }
+  }
+  double v[2] = {apr, apq}, tot[2];
+  if (grid_reduce<2>(v, partials, &sc->ctr[CTR_SPMVP], tot, sc->strict)) {
+    sc->pr = tot[0];
+    sc->pq = tot[1];
+    sc->alpha = tot[0] / tot[1];
+  }
+}
+
+// k_resid_replace in stream form.
+__global__ __launch_bounds__(256) void k_resid_replace_s(int n, const int *__restrict__ rp,
+                                                         const int *__restrict__ ci,
+                                                         const double *__restrict__ av,
+                                                         const double *__restrict__ x,
+                                                         const double *__restrict__ bvec,
+                                                         double *__restrict__ r,
+                                                         double *partials, DevScalars *sc) {
+  if (sc->done || sc->resid_period <= 0 || (sc->k % sc->resid_period) != 0) return;
+  __shared__ double sbuf[4][64 * kSpmvStream];
+  double *buf = sbuf[(threadIdx.x >> 6) & 3];
+  const int lane = threadIdx.x & 63;
+  const RowWalk rw = xcd_rows(n);
+  double arr = 0.0;
+  for (int r0 = rw.first; r0 < rw.hi; r0 += rw.step) {
+    const int row = r0 + lane;
+    const double bi = row < rw.hi ? bvec[row] : 0.0;
+    const double acc = stream_rows(r0, rw.hi, rp, ci, av, [&](int j) { return x[j]; }, buf);
+    if (row < rw.hi) {
+      const double ri = bi - acc;
+      r[row] = ri;
+      arr += ri * ri;
+    }
+  }
+  double v[1] = {arr}, tot[1];
+  if (grid_reduce<1>(v, partials, &sc->ctr[CTR_GEN], tot, sc->strict)) {
+    sc->rr = tot[0];
+    sc->done = !(sqrt(tot[0]) > sqrt(sc->bb) * sc->tol && sc->k < sc->maxit);
+  }
+}
+
+// k_spmv_res in stream form.
+__global__ __launch_bounds__(256) void k_spmv_res_s(int n, const int *__restrict__ rp,
+                                                    const int *__restrict__ ci,
+                                                    const double *__restrict__ av,
+                                                    const double *__restrict__ x,
+                                                    const double *__restrict__ bvec,
+                                                    double *__restrict__ y, double *partials,
+                                                    DevScalars *sc) {
+  __shared__ double sbuf[4][64 * kSpmvStream];
+  double *buf = sbuf[(threadIdx.x >> 6) & 3];
+  const int lane = threadIdx.x & 63;
+  const RowWalk rw = xcd_rows(n);
+  double aqq = 0.0;
+  for (int r0 = rw.first; r0 < rw.hi; r0 += rw.step) {
+    const int row = r0 + lane;
+    const double bi = (bvec && row < rw.hi) ? bvec[row] : 0.0;
+    double acc = stream_rows(r0, rw.hi, rp, ci, av, [&](int j) { return x[j]; }, buf);
+    if (row < rw.hi) {
+      acc -= bi;
+      y[row] = acc;
+      aqq += acc * acc;
+    }
+  }
+  if (bvec) {
+    double v[1] = {aqq}, tot[1];
+    if (grid_reduce<1>(v, partials, &sc->ctr[CTR_RES], tot, sc->strict)) {
       sc->qq = tot[0];
       sc->relres = sqrt(tot[0]) / sqrt(sc->bb);
     }

@@ -21,6 +21,8 @@ OPT_TRSV_LANES = 4
 OPT_PROFILE = 5
 OPT_SERIAL_WIDTH = 6
 OPT_COHERENT_GATHER = 7
+OPT_TASK_WAVES = 8
+OPT_STRICT_SYNC = 9
 
 
 class Stats(ctypes.Structure):

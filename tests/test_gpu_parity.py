@@ -55,20 +55,22 @@ def test_precond_golden(name):
 
 
 @pytest.mark.parametrize("name", FIXTURES)
-def test_spmv_golden(name):
+@pytest.mark.parametrize("lanes", [0, 8])
+def test_spmv_golden(name, lanes):
     A, Gs, d = load(name)
-    print(f"case spmv_golden {name}", flush=True)
-    with rchol_amd.Solver(A, Gs) as s:
+    print(f"case spmv_golden {name} lanes={lanes}", flush=True)
+    with rchol_amd.Solver(A, Gs, spmv_lanes=lanes) as s:
         y = s.spmv(d["b"])
         assert rel(y, d["Ab"]) <= SPMV_RTOL
 
 
 @pytest.mark.parametrize("name", FIXTURES)
 @pytest.mark.parametrize("graph_iters", [0, 16])
-def test_pcg_golden(name, graph_iters):
+@pytest.mark.parametrize("lanes", [0, 8])
+def test_pcg_golden(name, graph_iters, lanes):
     A, Gs, d = load(name)
-    print(f"case pcg_golden {name} graph_iters={graph_iters}", flush=True)
-    with rchol_amd.Solver(A, Gs, graph_iters=graph_iters) as s:
+    print(f"case pcg_golden {name} graph_iters={graph_iters} spmv_lanes={lanes}", flush=True)
+    with rchol_amd.Solver(A, Gs, graph_iters=graph_iters, spmv_lanes=lanes) as s:
         x, relres, itr = s.pcg(d["b"], float(d["tol"]), 5000)
     assert abs(itr - int(d["itr"])) <= 1
     assert relres <= float(d["tol"]) * 10
@@ -253,7 +255,7 @@ def test_autotune_keeps_results():
     print("case autotune", flush=True)
     with rchol_amd.Solver(A, Gs) as s:
         t = s.autotune(reps=2)
-        assert t["trsv_lanes"] in (1, 2, 4, 8) and t["spmv_lanes"] in (4, 8, 16)
+        assert t["trsv_lanes"] in (1, 2, 4, 8) and t["spmv_lanes"] in (0, 4, 8, 16)
         z = s.precond(d["b"])
         x, relres, itr = s.pcg(d["b"], 1e-10, 5000)
     assert rel(z, d["z"]) <= PRECOND_RTOL
@@ -270,6 +272,31 @@ def test_split_sweeps_match_oracle(name, width, precision):
         s.set_option(rchol_amd.solver.OPT_SERIAL_WIDTH, width)
         z = s.precond(d["b"])
         x, relres, itr = s.pcg(d["b"], 1e-10, 5000)
+    true = np.linalg.norm(A @ x - d["b"]) / np.linalg.norm(d["b"])
+    if precision == rchol_amd.FP64:
+        assert rel(z, d["z"]) <= PRECOND_RTOL
+        assert abs(itr - int(d["itr"])) <= 1
+    else:
+        assert rel(z, d["z"]) <= 1e-5
+        assert itr <= int(d["itr"]) + 5
+    assert true <= 1e-9 and relres == pytest.approx(true, rel=1e-6)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+@pytest.mark.parametrize("waves", [1, 3])
+@pytest.mark.parametrize("precision", [rchol_amd.FP64, rchol_amd.FP32])
+def test_stream_tasks_match_oracle(name, waves, precision):
+    """A task cut planned for very few waves turns the short rows of every
+    level into stream tasks (coalesced entry block, LDS row sums) and the
+    rest into multi-pass lane tasks."""
+    A, Gs, d = load(name)
+    print(f"case stream {name} waves={waves} prec={precision}", flush=True)
+    with rchol_amd.Solver(A, Gs, precision=precision) as s:
+        s.set_option(rchol_amd.solver.OPT_TASK_WAVES, waves)
+        z = s.precond(d["b"])
+        x, relres, itr = s.pcg(d["b"], 1e-10, 5000)
+        st = s.stats()
+    assert 0 < st["fwd_tasks"] <= A.shape[0] and 0 < st["bwd_tasks"] <= A.shape[0]
     true = np.linalg.norm(A @ x - d["b"]) / np.linalg.norm(d["b"])
     if precision == rchol_amd.FP64:
         assert rel(z, d["z"]) <= PRECOND_RTOL
