@@ -67,6 +67,22 @@ template <> struct Sent<float> {
   }
 };
 
+// Loads of data that a kernel reads exactly once (matrix indices and
+// values, per-slot metadata, the right-hand side): non-temporal, so that
+// the L2 keeps the gathered vector lines that neighbouring rows reuse
+// instead of filling with the streamed matrix.
+#ifndef RCHOL_AMD_NT
+#define RCHOL_AMD_NT 1
+#endif
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+#if RCHOL_AMD_NT
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+
 // Bound on one dependency wait, in wall-clock ticks (the constant-rate
 // counter runs at ~100 MHz: ~0.67 s), far beyond any legitimate wait.  A
 // broken schedule therefore ends as an error flag, never as a hang; once the
@@ -166,7 +182,8 @@ __device__ __forceinline__ void await_batch(const T *x, const int (&j)[U], T (&x
     jp = __shfl(jp, leader, 64);
     if (lane == leader) {
       for (int polls = 1;; polls++) {
-        for (int i = 0; i < backoff; i++) __builtin_amdgcn_s_sleep(1);
+        if (kPollCap > 0)
+          for (int i = 0; i < backoff; i++) __builtin_amdgcn_s_sleep(1);
         if (!Sent<T>::is(Sent<T>::load_agent(x + jp))) break;
         if (backoff < kPollCap) backoff <<= 1;
         if ((polls & 7) == 0 &&

@@ -59,8 +59,8 @@ __device__ __forceinline__ XT row_dot(int e0, int e1, int sub, const int *__rest
 #pragma unroll
     for (int u = 0; u < kGather; u++) {
       const int e = c + u * L;
-      j[u] = e < e1 ? ci[e] : -1;
-      v[u] = e < e1 ? cv[e] : VT(0);
+      j[u] = e < e1 ? ld_stream(ci + e) : -1;
+      v[u] = e < e1 ? ld_stream(cv + e) : VT(0);
     }
 #pragma unroll
     for (int u = 0; u < kGather; u++) {
@@ -101,7 +101,7 @@ __device__ __forceinline__ void sweep_task(int cnt, int m_row, int m_e0, int m_e
     const VT dg = __shfl(m_dg, k, 64);
     const bool live = k < cnt;
     if (!live) e1 = e0;
-    const BT b = (live && sub == 0) ? rhs[row] : (BT)0;
+    const BT b = (live && sub == 0) ? ld_stream(rhs + row) : (BT)0;
     XT acc = row_dot<VT, XT, L, true>(e0, e1, sub, ci, cv, x, err, coherent);
     acc = group_sum<L>(acc);
     if (live && sub == 0) {
@@ -138,7 +138,7 @@ __device__ __forceinline__ void sweep_stream(int cnt, int m_row, int m_e0, int m
   const int lane = threadIdx.x & 63;
   const bool live = lane < cnt;
   const int E0 = __shfl(m_e0, 0, 64), E1 = __shfl(m_e1, cnt - 1, 64);
-  const BT b = live ? rhs[m_row] : (BT)0;
+  const BT b = live ? ld_stream(rhs + m_row) : (BT)0;
   XT acc = 0;
   for (int c = E0; c < E1; c += 64 * kStream) {
     int j[kStream];
@@ -147,8 +147,8 @@ __device__ __forceinline__ void sweep_stream(int cnt, int m_row, int m_e0, int m
 #pragma unroll
     for (int u = 0; u < kStream; u++) {
       const int e = c + u * 64 + lane;
-      j[u] = e < E1 ? ci[e] : -1;
-      v[u] = e < E1 ? cv[e] : VT(0);
+      j[u] = e < E1 ? ld_stream(ci + e) : -1;
+      v[u] = e < E1 ? ld_stream(cv + e) : VT(0);
     }
 #pragma unroll
     for (int u = 0; u < kStream; u++) {
@@ -202,18 +202,30 @@ __global__ __launch_bounds__(256) void k_sptrsv(
   __shared__ XT sbuf[4][64 * kStream];  // one stream-task product buffer per wave
   XT *buf = sbuf[(threadIdx.x >> 6) & 3];
   const int k1 = wave_ptr[wave + 1];
-  for (int k = wave_ptr[wave]; k < k1; k++) {
-    const int s0 = wt_s0[k], info = wt_info[k];
+  // Software pipeline over the wave's list: the descriptor and per-slot
+  // metadata of task k+1 are loaded before task k runs, so a task starts
+  // with its rows, entry ranges and pivots already in registers.
+  int k = wave_ptr[wave];
+  int n_info = 0, n_row = 0, n_e0 = 0, n_e1 = 0;
+  VT n_dg = 1;
+  auto fetch = [&](int kk) {
+    const int s = wt_s0[kk];
+    n_info = wt_info[kk];
+    if (lane < (n_info & ((1 << kTaskInfoShift) - 1))) {
+      n_row = ld_stream(slot_row + s + lane);
+      n_e0 = ld_stream(rp + s + lane);
+      n_e1 = ld_stream(rp + s + lane + 1);
+      n_dg = ld_stream(diag + s + lane);
+    }
+  };
+  if (k < k1) fetch(k);
+  for (; k < k1; k++) {
+    const int info = n_info;
     const int lg = info >> kTaskInfoShift;
     const int cnt = info & ((1 << kTaskInfoShift) - 1);  // <= 64 slots
-    int m_row = 0, m_e0 = 0, m_e1 = 0;
-    VT m_dg = 1;
-    if (lane < cnt) {
-      m_row = slot_row[s0 + lane];
-      m_e0 = rp[s0 + lane];
-      m_e1 = rp[s0 + lane + 1];
-      m_dg = diag[s0 + lane];
-    }
+    const int m_row = n_row, m_e0 = n_e0, m_e1 = n_e1;
+    const VT m_dg = n_dg;
+    if (k + 1 < k1) fetch(k + 1);
 #define RCHOL_SWEEP_TASK(LL)                                                                  \
   sweep_task<VT, XT, BT, LL, FUSE>(cnt, m_row, m_e0, m_e1, m_dg, ci, cv, rhs, x, aux, dotv, err, \
                                    coherent, dacc)
@@ -597,7 +609,7 @@ __device__ __forceinline__ double stream_rows(int r0, int n, const int *__restri
   const int lane = threadIdx.x & 63;
   const int row = r0 + lane;
   const bool live = row < n;
-  const int e0 = live ? rp[row] : 0, e1 = live ? rp[row + 1] : 0;
+  const int e0 = live ? ld_stream(rp + row) : 0, e1 = live ? ld_stream(rp + row + 1) : 0;
   const int last = min(64, n - r0) - 1;
   const int E0 = __shfl(e0, 0, 64), E1 = __shfl(e1, last, 64);
   double acc = 0.0;
@@ -607,8 +619,8 @@ __device__ __forceinline__ double stream_rows(int r0, int n, const int *__restri
 #pragma unroll
     for (int u = 0; u < kSpmvStream; u++) {
       const int e = c + u * 64 + lane;
-      j[u] = e < E1 ? ci[e] : -1;
-      a[u] = e < E1 ? av[e] : 0.0;
+      j[u] = e < E1 ? ld_stream(ci + e) : -1;
+      a[u] = e < E1 ? ld_stream(av + e) : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < kSpmvStream; u++) pv[u] = j[u] >= 0 ? g(j[u]) : 0.0;

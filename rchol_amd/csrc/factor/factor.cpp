@@ -210,6 +210,10 @@ class Factorizer {
           if (x.row >= lb && x.row < lb + m && alive[x.row - lb]) blocked[x.row - lb] = 1;
       }
       for (size_t t = 0; t < nc; t++) blocked[cand[t]] = 0;
+      // the layer is chosen in priority order but numbered (and eliminated:
+      // its vertices are independent, so any order is valid) in ND order,
+      // so that consecutive positions stay neighbours in the domain
+      std::sort(layer.begin(), layer.end());
       for (size_t i : layer) {
         const size_t v = lb + i;
         // current neighbour list: live block vertices and outside vertices,
