@@ -45,7 +45,7 @@ typedef struct rchol_amd_ctx rchol_amd_ctx;
 #define RCHOL_AMD_OPT_TRSV_LANES 4    /* minimum lanes per row in the triangular sweeps
                                          (power of two <= 64, default 1); a task
                                          widens to cover its longest row in one
-                                         round of 4 gathers per lane */
+                                         round of 8 gathers per lane */
 #define RCHOL_AMD_OPT_PROFILE 5       /* 1: time every kernel of the loop with HIP events */
 #define RCHOL_AMD_OPT_SERIAL_WIDTH 6  /* leading/trailing sweep levels of at most this many
                                          rows run in one workgroup (0 = off) */

@@ -194,6 +194,7 @@ __device__ __forceinline__ void await_batch(const T *x, const int (&j)[U], T (&x
         }
       }
     }
+    backoff = 1;  // progress: the next wait is likely short
     pend = false;
 #pragma unroll
     for (int u = 0; u < U; u++)

@@ -40,7 +40,7 @@ namespace rchol_amd {
 //            sc->rz; the last block forms beta (pcg.cpp:93-95).
 // ---------------------------------------------------------------------------
 #ifndef RCHOL_AMD_GATHER
-#define RCHOL_AMD_GATHER 4
+#define RCHOL_AMD_GATHER 8
 #endif
 constexpr int kGather = RCHOL_AMD_GATHER;  // gathers in flight per lane and round
 
