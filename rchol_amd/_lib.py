@@ -111,10 +111,25 @@ def _bind(lib, table):
         f.argtypes = args
 
 
+def _share_torch_runtime():
+    """Import PyTorch, when it is installed, before the engine library.
+
+    PyTorch-ROCm ships its own HIP/HSA runtime, which it requests under the
+    unversioned name libamdhip64.so.  The engine links libamdhip64.so.7: loaded
+    after torch it binds to torch's copy (same SONAME) and one runtime serves
+    both; loaded before torch, torch would start a second runtime in the
+    process, which finds no GPU.  The engine itself does not use torch."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def engine():
     """The HIP engine library (loaded once)."""
     global _engine
     if _engine is None:
+        _share_torch_runtime()
         if not os.path.exists(ENGINE_PATH):
             raise RuntimeError(
                 f"{ENGINE_PATH} is not built; run `make -C rchol_amd/csrc` "
