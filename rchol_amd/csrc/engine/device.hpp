@@ -160,12 +160,16 @@ constexpr int kPollCap = RCHOL_AMD_POLL_CAP;
 // pending slots at once (one load latency for the whole round).  A wait
 // longer than kWaitTicks raises the error flag; a raised flag zeroes what is
 // still pending and returns.
+// Slot u of a lane refers to x[j[u] * stride + col] (stride / col: the
+// interleaved columns of a multi-right-hand-side block; 1 / 0 otherwise).
 template <typename T, int U>
-__device__ __forceinline__ void await_batch(const T *x, const int (&j)[U], T (&xv)[U], int *err) {
+__device__ __forceinline__ void await_batch(const T *x, const int (&j)[U], T (&xv)[U], int *err,
+                                            int stride = 1, int col = 0) {
+  x += col;
   bool pend = false;
 #pragma unroll
   for (int u = 0; u < U; u++)
-    if (j[u] >= 0 && Sent<T>::is(xv[u])) xv[u] = Sent<T>::load_agent(x + j[u]);
+    if (j[u] >= 0 && Sent<T>::is(xv[u])) xv[u] = Sent<T>::load_agent(x + (size_t)j[u] * stride);
 #pragma unroll
   for (int u = 0; u < U; u++) pend |= j[u] >= 0 && Sent<T>::is(xv[u]);
   unsigned long long mask = __ballot(pend);
@@ -184,7 +188,7 @@ __device__ __forceinline__ void await_batch(const T *x, const int (&j)[U], T (&x
       for (int polls = 1;; polls++) {
         if (kPollCap > 0)
           for (int i = 0; i < backoff; i++) __builtin_amdgcn_s_sleep(1);
-        if (!Sent<T>::is(Sent<T>::load_agent(x + jp))) break;
+        if (!Sent<T>::is(Sent<T>::load_agent(x + (size_t)jp * stride))) break;
         if (backoff < kPollCap) backoff <<= 1;
         if ((polls & 7) == 0 &&
             (wall_clock64() - t0 > kWaitTicks ||
@@ -198,7 +202,7 @@ __device__ __forceinline__ void await_batch(const T *x, const int (&j)[U], T (&x
     pend = false;
 #pragma unroll
     for (int u = 0; u < U; u++)
-      if (j[u] >= 0 && Sent<T>::is(xv[u])) xv[u] = Sent<T>::load_agent(x + j[u]);
+      if (j[u] >= 0 && Sent<T>::is(xv[u])) xv[u] = Sent<T>::load_agent(x + (size_t)j[u] * stride);
 #pragma unroll
     for (int u = 0; u < U; u++) pend |= j[u] >= 0 && Sent<T>::is(xv[u]);
     mask = __ballot(pend);

@@ -189,6 +189,43 @@ void cut_tasks(const std::vector<int> &lp, const std::vector<int> &rp, int h, in
   }
 }
 
+// Tasks for the K-column sweeps: every level cut into runs of rows with an
+// entry-lane width Le = 2^lg <= 64/K (a row takes K * Le lanes) that covers
+// the longest of the run's rows in one round of kGather gathers, R = 64/(K Le)
+// rows per pass; narrow levels get one-pass tasks, wide ones as many passes
+// (at most 64 slots) as it takes to need about one task per wave.
+void cut_tasks_mr(const std::vector<int> &lp, const std::vector<int> &rp, int nlevels, int K,
+                  long long nwaves, TaskCut &tc) {
+  tc.s0.clear();
+  tc.info.clear();
+  tc.level.clear();
+  int maxlg = 0;
+  while ((K << (maxlg + 1)) <= 64) maxlg++;
+  for (int l = 0; l < nlevels; l++) {
+    const int s_end = lp[l + 1];
+    const long long rows = s_end - lp[l];
+    for (int s = lp[l]; s < s_end;) {
+      int lg = std::min(lane_lg(rp[s + 1] - rp[s], 0), maxlg), take = 0;
+      for (;;) {
+        const int R = 64 / (K << lg);
+        const long long one_pass = (rows + R - 1) / R;
+        const int passes =
+            (int)std::min<long long>(64 / R, std::max<long long>(1, (one_pass + nwaves - 1) / nwaves));
+        take = std::min(s_end - s, passes * R);
+        int need = lg;
+        for (int k = 0; k < take; k++)
+          need = std::max(need, std::min(lane_lg(rp[s + k + 1] - rp[s + k], 0), maxlg));
+        if (need == lg) break;
+        lg = need;
+      }
+      tc.s0.push_back(s);
+      tc.info.push_back(take | (lg << kTaskInfoShift));
+      tc.level.push_back(l);
+      s += take;
+    }
+  }
+}
+
 // Static task -> wave assignment.  Block b runs on XCD b % 8 (round-robin
 // dispatch), so a level's tasks are split into 8 contiguous runs, run x
 // going to the waves of XCD x: neighbouring rows, and the rows of the same
@@ -298,6 +335,7 @@ struct rchol_amd_ctx {
   int coherent_gather = 0;  // sweeps read x with agent-scope loads from the start
   int task_waves = 0;       // waves the task cut plans for (0 = resident capacity)
   int strict_sync = 0;      // grid reductions with release/acquire fences
+  long long g_gen = 0;      // bumped by every upload_G (keys the batched schedule)
   std::vector<hipEvent_t> pev;  // 5 per captured iteration when profiling
   // captured iteration graph
   hipGraph_t graph = nullptr;
@@ -315,6 +353,10 @@ struct rchol_amd_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int grid = 0;
+    // sweep schedule cut for K columns: per-wave task lists, [0] fwd, [1] bwd
+    DevBuf<int> wave_ptr[2], wt_s0[2], wt_info[2];
+    int sched_K = 0, ntasks[2] = {0, 0};
+    long long sched_gen = -1;
     void drop_graph() {
       if (gexec) (void)hipGraphExecDestroy(gexec);
       if (graph) (void)hipGraphDestroy(graph);
@@ -383,6 +425,7 @@ struct rchol_amd_ctx {
   int precond(const double *d_r, double *d_z);
   // multi-RHS
   int ensure_batch(int K);
+  int ensure_batch_schedule(int K, int per_cu);
   template <int K>
   int pcg_batch(const double *d_b, int ncols, double tol, int maxit, double *d_x, int keep_x,
                 double *relres, int *itr);
@@ -753,7 +796,8 @@ static int trsv_occupancy_mr() {
 }
 
 int rchol_amd_ctx::ensure_batch(int K) {
-  if (batch.K == K && batch.n == n && batch.b.p) return RCHOL_AMD_OK;
+  if (batch.K == K && batch.n == n && batch.b.p && batch.sched_K == K && batch.sched_gen == g_gen)
+    return RCHOL_AMD_OK;
   batch.drop_graph();
   const size_t m = n * (size_t)K;
   HIP_TRY(batch.b.alloc(m));
@@ -790,10 +834,35 @@ int rchol_amd_ctx::ensure_batch(int K) {
   }
   if (per_cu <= 0) return fail(RCHOL_AMD_EHIP, "occupancy query for the batched sweeps failed");
   per_cu = resident_blocks(per_cu);
-  const int max_tasks = std::max(fwd.ntasks, bwd.ntasks);
-  batch.grid = std::max(1, std::min(per_cu * ncu, (max_tasks + 3) / 4));
   batch.K = K;
   batch.n = n;
+  return ensure_batch_schedule(K, per_cu);
+}
+
+// The K-column sweep schedule for the current G (rebuilt after every
+// upload_G or change of K).
+int rchol_amd_ctx::ensure_batch_schedule(int K, int per_cu) {
+  if (batch.sched_K == K && batch.sched_gen == g_gen) return RCHOL_AMD_OK;
+  const int cap_blocks = per_cu * ncu;
+  const long long plan_waves = (long long)cap_blocks * (kBlock / 64);
+  TaskCut tc[2];
+  Sweep *sw[2] = {&fwd, &bwd};
+  size_t max_tasks = 0;
+  for (int k = 0; k < 2; k++) {
+    cut_tasks_mr(sw[k]->level_ptr_h, sw[k]->rp_h, sw[k]->nlevels, K, plan_waves, tc[k]);
+    max_tasks = std::max(max_tasks, tc[k].s0.size());
+  }
+  batch.grid = (int)std::max<size_t>(1, std::min<size_t>(cap_blocks, (max_tasks + 3) / 4));
+  for (int k = 0; k < 2; k++) {
+    std::vector<int> wp, ws0, winfo;
+    assign_waves(tc[k], batch.grid, wp, ws0, winfo);
+    HIP_TRY(batch.wave_ptr[k].upload(wp.data(), wp.size()));
+    HIP_TRY(batch.wt_s0[k].upload(ws0.data(), ws0.size()));
+    HIP_TRY(batch.wt_info[k].upload(winfo.data(), winfo.size()));
+    batch.ntasks[k] = (int)tc[k].s0.size();
+  }
+  batch.sched_K = K;
+  batch.sched_gen = g_gen;
   return RCHOL_AMD_OK;
 }
 
@@ -801,39 +870,38 @@ template <int K>
 void rchol_amd_ctx::launch_sweeps_mr(const double *rhs, bool with_dot, const int *stop,
                                      hipEvent_t *ev) {
   const dim3 g(batch.grid), blk(kBlock);
+  const int *wpf = batch.wave_ptr[0].p, *wsf = batch.wt_s0[0].p, *wif = batch.wt_info[0].p;
+  const int *wpb = batch.wave_ptr[1].p, *wsb = batch.wt_s0[1].p, *wib = batch.wt_info[1].p;
   if (gprec == RCHOL_AMD_FP32) {
-    hipLaunchKernelGGL((k_sptrsv_mr<float, float, double, K, 1>), g, blk, 0, stream, fwd.ntasks,
-                       fwd.task_ptr.p, fwd.slot_row.p, fwd.rp.p, fwd.ci.p, fwd.cv32.p,
-                       fwd.diag32.p, rhs, batch.y32.p, batch.z32.p, nullptr, nullptr, batch.bs,
-                       stop);
+    hipLaunchKernelGGL((k_sptrsv_mr<float, float, double, K, 1>), g, blk, 0, stream, wpf, wsf,
+                       wif, fwd.slot_row.p, fwd.rp.p, fwd.ci.p, fwd.cv32.p, fwd.diag32.p, rhs,
+                       batch.y32.p, batch.z32.p, nullptr, nullptr, batch.bs, stop);
     if (ev) (void)hipEventRecord(ev[1], stream);
     if (with_dot)
-      hipLaunchKernelGGL((k_sptrsv_mr<float, float, float, K, 2>), g, blk, 0, stream, bwd.ntasks,
-                         bwd.task_ptr.p, bwd.slot_row.p, bwd.rp.p, bwd.ci.p, bwd.cv32.p,
-                         bwd.diag32.p, (const float *)batch.y32.p, batch.z32.p, (float *)nullptr,
+      hipLaunchKernelGGL((k_sptrsv_mr<float, float, float, K, 2>), g, blk, 0, stream, wpb, wsb,
+                         wib, bwd.slot_row.p, bwd.rp.p, bwd.ci.p, bwd.cv32.p, bwd.diag32.p,
+                         (const float *)batch.y32.p, batch.z32.p, (float *)nullptr,
                          (const double *)batch.r.p, batch.partials.p, batch.bs, stop);
     else
-      hipLaunchKernelGGL((k_sptrsv_mr<float, float, float, K, 0>), g, blk, 0, stream, bwd.ntasks,
-                         bwd.task_ptr.p, bwd.slot_row.p, bwd.rp.p, bwd.ci.p, bwd.cv32.p,
-                         bwd.diag32.p, (const float *)batch.y32.p, batch.z32.p, (float *)nullptr,
-                         nullptr, nullptr, batch.bs, stop);
+      hipLaunchKernelGGL((k_sptrsv_mr<float, float, float, K, 0>), g, blk, 0, stream, wpb, wsb,
+                         wib, bwd.slot_row.p, bwd.rp.p, bwd.ci.p, bwd.cv32.p, bwd.diag32.p,
+                         (const float *)batch.y32.p, batch.z32.p, (float *)nullptr, nullptr,
+                         nullptr, batch.bs, stop);
   } else {
-    hipLaunchKernelGGL((k_sptrsv_mr<double, double, double, K, 1>), g, blk, 0, stream,
-                       fwd.ntasks, fwd.task_ptr.p, fwd.slot_row.p, fwd.rp.p, fwd.ci.p,
-                       fwd.cv64.p, fwd.diag64.p, rhs, batch.y64.p, batch.z64.p, nullptr, nullptr,
-                       batch.bs, stop);
+    hipLaunchKernelGGL((k_sptrsv_mr<double, double, double, K, 1>), g, blk, 0, stream, wpf, wsf,
+                       wif, fwd.slot_row.p, fwd.rp.p, fwd.ci.p, fwd.cv64.p, fwd.diag64.p, rhs,
+                       batch.y64.p, batch.z64.p, nullptr, nullptr, batch.bs, stop);
     if (ev) (void)hipEventRecord(ev[1], stream);
     if (with_dot)
-      hipLaunchKernelGGL((k_sptrsv_mr<double, double, double, K, 2>), g, blk, 0, stream,
-                         bwd.ntasks, bwd.task_ptr.p, bwd.slot_row.p, bwd.rp.p, bwd.ci.p,
-                         bwd.cv64.p, bwd.diag64.p, (const double *)batch.y64.p, batch.z64.p,
-                         (double *)nullptr, (const double *)batch.r.p, batch.partials.p,
-                         batch.bs, stop);
+      hipLaunchKernelGGL((k_sptrsv_mr<double, double, double, K, 2>), g, blk, 0, stream, wpb,
+                         wsb, wib, bwd.slot_row.p, bwd.rp.p, bwd.ci.p, bwd.cv64.p, bwd.diag64.p,
+                         (const double *)batch.y64.p, batch.z64.p, (double *)nullptr,
+                         (const double *)batch.r.p, batch.partials.p, batch.bs, stop);
     else
-      hipLaunchKernelGGL((k_sptrsv_mr<double, double, double, K, 0>), g, blk, 0, stream,
-                         bwd.ntasks, bwd.task_ptr.p, bwd.slot_row.p, bwd.rp.p, bwd.ci.p,
-                         bwd.cv64.p, bwd.diag64.p, (const double *)batch.y64.p, batch.z64.p,
-                         (double *)nullptr, nullptr, nullptr, batch.bs, stop);
+      hipLaunchKernelGGL((k_sptrsv_mr<double, double, double, K, 0>), g, blk, 0, stream, wpb,
+                         wsb, wib, bwd.slot_row.p, bwd.rp.p, bwd.ci.p, bwd.cv64.p, bwd.diag64.p,
+                         (const double *)batch.y64.p, batch.z64.p, (double *)nullptr, nullptr,
+                         nullptr, batch.bs, stop);
   }
   if (ev) (void)hipEventRecord(ev[2], stream);
 }
@@ -846,8 +914,9 @@ void rchol_amd_ctx::launch_iteration_mr(int parity, int slot) {
   if (ev) (void)hipEventRecord(ev[0], stream);
   launch_sweeps_mr<K>(batch.r.p, true, &batch.bs->all_done, ev);
   const int rows_per_block = (kBlock / 64) * (64 / K);
-  const int sg = (int)std::max<size_t>(
+  int sg = (int)std::max<size_t>(
       1, std::min<size_t>((n + rows_per_block - 1) / rows_per_block, (size_t)ncu * 8));
+  if (sg > 8) sg = (sg + 7) / 8 * 8;  // whole XCD rounds for the row walk
   const int vg = (int)std::max<size_t>(
       1, std::min<size_t>((n * K + kBlock - 1) / kBlock, (size_t)ncu * 8));
   if (gprec == RCHOL_AMD_FP32) {
@@ -884,6 +953,8 @@ int rchol_amd_ctx::pcg_batch(const double *d_b, int ncols, double tol, int maxit
   batch.hbs->maxit = maxit;
   batch.hbs->ncols = ncols;
   batch.hbs->zero = 0;
+  batch.hbs->strict = strict_sync;
+  HIP_TRY(hipMemcpyAsync(&batch.bs->strict, &batch.hbs->strict, sizeof(int), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(&batch.bs->tol, &batch.hbs->tol, sizeof(double), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(&batch.bs->maxit, &batch.hbs->maxit, sizeof(int), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(&batch.bs->ncols, &batch.hbs->ncols, sizeof(int), hipMemcpyHostToDevice, stream));
@@ -947,8 +1018,9 @@ int rchol_amd_ctx::pcg_batch(const double *d_b, int ncols, double tol, int maxit
   if (batch.hbs->err)
     return fail(RCHOL_AMD_ESTALL, "triangular sweep wait exceeded its bound (err=%d)", batch.hbs->err);
   const int rows_per_block = (kBlock / 64) * (64 / K);
-  const int sg = (int)std::max<size_t>(
+  int sg = (int)std::max<size_t>(
       1, std::min<size_t>((n + rows_per_block - 1) / rows_per_block, (size_t)ncu * 8));
+  if (sg > 8) sg = (sg + 7) / 8 * 8;  // whole XCD rounds for the row walk
   hipLaunchKernelGGL((k_spmv_res_mr<K>), dim3(sg), dim3(kBlock), 0, stream, (int)n, arp.p, aci.p,
                      av.p, (const double *)batch.x.p, d_b, batch.res.p, batch.partials.p, batch.bs);
   HIP_TRY(hipGetLastError());
@@ -1413,6 +1485,7 @@ int rchol_amd_upload_G(rchol_amd_ctx *c, size_t n, const size_t *colPtr, const s
   c->nG = n;
   c->nnzG = nnz;
   c->gprec = precision;
+  c->g_gen++;
   rc = c->size_trsv_grid();
   if (rc) return rc;
   c->hasG = true;

@@ -5,7 +5,7 @@
 // column runs its own PCG recurrence (pcg.cpp:57-127) with its own scalars
 // and convergence mask; a converged column is frozen.
 #pragma once
-#include "device.hpp"
+#include "kernels.hpp"
 
 namespace rchol_amd {
 
@@ -22,6 +22,7 @@ struct BatchScalars {
   int maxit;
   int err;
   int zero;
+  int strict;  // 1: reductions publish partials with release/acquire fences
   double tol;
   unsigned int ctr[8];
 };
@@ -37,10 +38,13 @@ __device__ __forceinline__ double column_sum(double v) {
 // Per-column block sums, then the last block sums every column over all
 // blocks in block order (deterministic).  Returns true in every thread of the
 // last block (block-uniform); threads c < K hold the total of column c in tot.
-// Two reductions in one kernel must use disjoint partial regions.
+// Two reductions in one kernel must use disjoint partial regions.  The
+// publication handshake is grid_reduce's: value-returning atomic exchanges
+// and a relaxed counter (strict == 0), or plain stores and an
+// acquire-release counter (strict == 1).
 template <int K>
 __device__ __forceinline__ bool column_grid_reduce(double v, double *partials, unsigned int *ctr,
-                                                   double &tot) {
+                                                   double &tot, int strict) {
   __shared__ double red[16][K];
   __shared__ int is_last;
   v = column_sum<K>(v);
@@ -50,20 +54,32 @@ __device__ __forceinline__ bool column_grid_reduce(double v, double *partials, u
   if (threadIdx.x < K) {
     double s = 0.0;
     for (int i = 0; i < nw; i++) s += red[i][threadIdx.x];
-    partials[(size_t)blockIdx.x * K + threadIdx.x] = s;
+    double *slot = partials + (size_t)blockIdx.x * K + threadIdx.x;
+    if (strict) {
+      *slot = s;
+    } else {
+      const unsigned long long got = __hip_atomic_exchange(
+          (unsigned long long *)slot, (unsigned long long)__double_as_longlong(s),
+          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      asm volatile("" ::"v"(got) : "memory");
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned int prev =
-        __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        strict ? __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT)
+               : __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     is_last = (prev == gridDim.x - 1);
   }
   __syncthreads();
   if (!is_last) return false;
-  __atomic_thread_fence(__ATOMIC_ACQUIRE);
+  if (strict) __atomic_thread_fence(__ATOMIC_ACQUIRE);
   if (threadIdx.x < K) {
     double s = 0.0;
-    for (unsigned int b = 0; b < gridDim.x; b++) s += partials[(size_t)b * K + threadIdx.x];
+    for (unsigned int b = 0; b < gridDim.x; b++) {
+      const double *slot = partials + (size_t)b * K + threadIdx.x;
+      s += strict ? *slot : Sent<double>::load_agent(slot);
+    }
     tot = s;
   }
   __syncthreads();
@@ -83,59 +99,124 @@ __device__ __forceinline__ void set_all_done(BatchScalars *bs) {
   }
 }
 
-// Level-scheduled sweep on K interleaved columns (same schedule and
-// deadlock-freedom argument as k_sptrsv); a wave holds 64/K rows.
+// Level-scheduled sweep on K interleaved columns (the deadlock-freedom
+// argument of k_sptrsv; per-wave task lists cut for K columns).  A row
+// takes K * Le lanes: lane (grp, sub, c) = lane / (K Le), lane / K % Le,
+// lane % K owns column c of entries e0 + sub, + Le, ... of row grp of the
+// pass, so one index/value load serves K columns and every gather is one
+// contiguous K-wide segment.  A round issues kGather gathers per lane
+// before looking at any; pending values are awaited wave-cooperatively.
 // FUSE 1: sentinel into aux; FUSE 2: per-column dotv.x -> rz, beta.
-template <typename VT, typename XT, typename BT, int K, int FUSE>
-__global__ __launch_bounds__(256) void k_sptrsv_mr(
-    int ntasks, const int *__restrict__ task_ptr, const int *__restrict__ slot_row,
-    const int *__restrict__ rp, const int *__restrict__ ci, const VT *__restrict__ cv,
-    const VT *__restrict__ diag, const BT *__restrict__ rhs, XT *x, XT *__restrict__ aux,
-    const double *__restrict__ dotv, double *__restrict__ partials, BatchScalars *bs,
-    const int *stop) {
-  if (*stop) return;
-  constexpr int R = 64 / K;
-  const int lane = threadIdx.x & 63;
-  const int grp = lane / K, c = lane % K;
-  const int nwaves = gridDim.x * (blockDim.x >> 6);
-  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  int *err = &bs->err;
-  double dacc = 0.0;
-  for (int t = wave; t < ntasks; t += nwaves) {
-    const int s0 = task_ptr[t], s1 = task_ptr[t + 1];
-    const int cnt = s1 - s0;  // <= 64 slots: one coalesced metadata load
-    int m_row = 0, m_e0 = 0, m_e1 = 0;
-    VT m_dg = 1;
-    if (lane < cnt) {
-      m_row = slot_row[s0 + lane];
-      m_e0 = rp[s0 + lane];
-      m_e1 = rp[s0 + lane + 1];
-      m_dg = diag[s0 + lane];
+template <typename VT, typename XT, int Le, int K>
+__device__ __forceinline__ XT row_dot_mr(int e0, int e1, int sub, int c,
+                                         const int *__restrict__ ci, const VT *__restrict__ cv,
+                                         const XT *x, int *err) {
+  XT acc = 0;
+  for (int b0 = e0 + sub; b0 < e1; b0 += kGather * Le) {
+    int j[kGather];
+    VT v[kGather];
+    XT xv[kGather];
+#pragma unroll
+    for (int u = 0; u < kGather; u++) {
+      const int e = b0 + u * Le;
+      j[u] = e < e1 ? ld_stream(ci + e) : -1;
+      v[u] = e < e1 ? ld_stream(cv + e) : VT(0);
     }
-    for (int base = 0; base < cnt; base += R) {
-      const int kk = base + grp;
-      const int row = __shfl(m_row, kk, 64);
-      const int e0 = __shfl(m_e0, kk, 64), e1 = __shfl(m_e1, kk, 64);
-      const VT dg = __shfl(m_dg, kk, 64);
-      if (kk >= cnt) continue;
-      const size_t o = (size_t)row * K + c;
-      const BT b = rhs[o];
-      XT acc = 0;
-#pragma unroll 4
-      for (int e = e0; e < e1; e++) {
-        const int j = ci[e];
-        const XT xj = wait_ready<XT>(x + (size_t)j * K + c, err);
-        acc += (XT)cv[e] * xj;
-      }
+#pragma unroll
+    for (int u = 0; u < kGather; u++) xv[u] = j[u] < 0 ? XT(0) : x[(size_t)j[u] * K + c];
+    bool pend = false;
+#pragma unroll
+    for (int u = 0; u < kGather; u++) pend |= Sent<XT>::is(xv[u]);
+    if (__ballot(pend)) await_batch<XT, kGather>(x, j, xv, err, K, c);
+#pragma unroll
+    for (int u = 0; u < kGather; u++) acc += (XT)v[u] * xv[u];
+  }
+  return acc;
+}
+
+template <typename VT, typename XT, typename BT, int K, int Le, int FUSE>
+__device__ __forceinline__ void sweep_task_mr(int cnt, int m_row, int m_e0, int m_e1, VT m_dg,
+                                              const int *__restrict__ ci,
+                                              const VT *__restrict__ cv,
+                                              const BT *__restrict__ rhs, XT *x,
+                                              XT *__restrict__ aux,
+                                              const double *__restrict__ dotv, int *err,
+                                              double &dacc) {
+  constexpr int W = K * Le, R = 64 / W;
+  const int lane = threadIdx.x & 63;
+  const int c = lane % K, sub = (lane / K) % Le, grp = lane / W;
+  for (int base = 0; base < cnt; base += R) {
+    const int k = base + grp;  // < 64
+    const int row = __shfl(m_row, k, 64);
+    int e0 = __shfl(m_e0, k, 64), e1 = __shfl(m_e1, k, 64);
+    const VT dg = __shfl(m_dg, k, 64);
+    const bool live = k < cnt;
+    if (!live) e1 = e0;
+    const size_t o = (size_t)row * K + c;
+    const BT b = (live && sub == 0) ? ld_stream(rhs + o) : (BT)0;
+    XT acc = row_dot_mr<VT, XT, Le, K>(e0, e1, sub, c, ci, cv, x, err);
+#pragma unroll
+    for (int off = K; off < W; off <<= 1) acc += __shfl_xor(acc, off, 64);
+    if (live && sub == 0) {
       const XT v = ((XT)b - acc) / (XT)dg;
       Sent<XT>::store_agent(x + o, v);
       if (FUSE == 1) aux[o] = Sent<XT>::value();
       if (FUSE == 2) dacc += dotv[o] * (double)v;
     }
   }
+}
+
+template <typename VT, typename XT, typename BT, int K, int FUSE>
+__global__ __launch_bounds__(256) void k_sptrsv_mr(
+    const int *__restrict__ wave_ptr, const int *__restrict__ wt_s0,
+    const int *__restrict__ wt_info, const int *__restrict__ slot_row,
+    const int *__restrict__ rp, const int *__restrict__ ci, const VT *__restrict__ cv,
+    const VT *__restrict__ diag, const BT *__restrict__ rhs, XT *x, XT *__restrict__ aux,
+    const double *__restrict__ dotv, double *__restrict__ partials, BatchScalars *bs,
+    const int *stop) {
+  if (*stop) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  int *err = &bs->err;
+  double dacc = 0.0;
+  const int k1 = wave_ptr[wave + 1];
+  int k = wave_ptr[wave];
+  int n_info = 0, n_row = 0, n_e0 = 0, n_e1 = 0;
+  VT n_dg = 1;
+  auto fetch = [&](int kk) {
+    const int s = wt_s0[kk];
+    n_info = wt_info[kk];
+    if (lane < (n_info & ((1 << kTaskInfoShift) - 1))) {
+      n_row = ld_stream(slot_row + s + lane);
+      n_e0 = ld_stream(rp + s + lane);
+      n_e1 = ld_stream(rp + s + lane + 1);
+      n_dg = ld_stream(diag + s + lane);
+    }
+  };
+  if (k < k1) fetch(k);
+  for (; k < k1; k++) {
+    const int info = n_info;
+    const int lg = info >> kTaskInfoShift;
+    const int cnt = info & ((1 << kTaskInfoShift) - 1);
+    const int m_row = n_row, m_e0 = n_e0, m_e1 = n_e1;
+    const VT m_dg = n_dg;
+    if (k + 1 < k1) fetch(k + 1);
+#define RCHOL_MR_TASK(LE)                                                                  \
+  sweep_task_mr<VT, XT, BT, K, ((LE) * K <= 64 ? (LE) : 64 / K), FUSE>(                    \
+      cnt, m_row, m_e0, m_e1, m_dg, ci, cv, rhs, x, aux, dotv, err, dacc)
+    switch (lg) {
+      case 0: RCHOL_MR_TASK(1); break;
+      case 1: RCHOL_MR_TASK(2); break;
+      case 2: RCHOL_MR_TASK(4); break;
+      case 3: RCHOL_MR_TASK(8); break;
+      default: RCHOL_MR_TASK(16); break;
+    }
+#undef RCHOL_MR_TASK
+  }
   if (FUSE == 2) {
     double tot = 0.0;
-    if (column_grid_reduce<K>(dacc, partials, &bs->ctr[CTR_BWD], tot) && threadIdx.x < K) {
+    if (column_grid_reduce<K>(dacc, partials, &bs->ctr[CTR_BWD], tot, bs->strict) &&
+        threadIdx.x < K) {
       const int cc = threadIdx.x;
       bs->rz[cc] = tot;
       bs->beta[cc] = (bs->k[cc] > 0) ? tot / bs->rz_old[cc] : 0.0;
@@ -160,7 +241,7 @@ __global__ __launch_bounds__(256) void k_init_mr(int n, const double *__restrict
     acc += bi * bi;
   }
   double tot = 0.0;
-  if (column_grid_reduce<K>(acc, partials, &bs->ctr[CTR_INIT], tot)) {
+  if (column_grid_reduce<K>(acc, partials, &bs->ctr[CTR_INIT], tot, bs->strict)) {
     if (threadIdx.x < K) {
       const int cc = threadIdx.x;
       bs->bb[cc] = tot;
@@ -173,6 +254,45 @@ __global__ __launch_bounds__(256) void k_init_mr(int n, const double *__restrict
     if (threadIdx.x == 0) bs->err = 0;
     set_all_done<K>(bs);
   }
+}
+
+// Rows of a multi-column SpMV: K lanes per row (lane c = column c), 64/K
+// rows per wave step, the XCD-contiguous walk of xcd_rows in steps of
+// 64/K rows.  Entries go in rounds of kGather per lane: all index/value
+// loads, then all K-wide gathers, then the products.
+template <int K>
+__device__ __forceinline__ RowWalk xcd_rows_mr(int n) {
+  constexpr int R = 64 / K;
+  const int X = (gridDim.x % 8 == 0) ? 8 : 1;
+  const int x = blockIdx.x % X, jb = blockIdx.x / X, nbx = gridDim.x / X;
+  const int wpb = blockDim.x >> 6;
+  const int lo = (int)((long long)n * x / X);
+  RowWalk w;
+  w.hi = (int)((long long)n * (x + 1) / X);
+  w.first = lo + (jb * wpb + (threadIdx.x >> 6)) * R + (threadIdx.x & 63) / K;
+  w.step = nbx * wpb * R;
+  return w;
+}
+
+template <int K, typename G>
+__device__ __forceinline__ double row_dot_a(int e0, int e1, const int *__restrict__ ci,
+                                            const double *__restrict__ av, G g) {
+  double acc = 0.0;
+  for (int b0 = e0; b0 < e1; b0 += kGather) {
+    int j[kGather];
+    double a[kGather], pv[kGather];
+#pragma unroll
+    for (int u = 0; u < kGather; u++) {
+      const int e = b0 + u;
+      j[u] = e < e1 ? ld_stream(ci + e) : -1;
+      a[u] = e < e1 ? ld_stream(av + e) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kGather; u++) pv[u] = j[u] >= 0 ? g(j[u]) : 0.0;
+#pragma unroll
+    for (int u = 0; u < kGather; u++) acc += a[u] * pv[u];
+  }
+  return acc;
 }
 
 // p_new = z + beta_c p_old (p = z at k_c == 0), q = A p_new, per-column
@@ -188,33 +308,32 @@ __global__ __launch_bounds__(256) void k_spmv_p_mr(int n, const int *__restrict_
                                                    const double *__restrict__ r,
                                                    double *partials, BatchScalars *bs) {
   if (bs->all_done) return;
-  constexpr int R = 64 / K;
-  const int lane = threadIdx.x & 63, grp = lane / K, c = lane % K;
+  const int c = (threadIdx.x & 63) % K;
   const bool first = bs->k[c] == 0;
   const double beta = bs->beta[c];
-  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int nw = gridDim.x * (blockDim.x >> 6);
+  auto pval = [&](int j) -> double {
+    const size_t o = (size_t)j * K + c;
+    return first ? (double)z[o] : beta * p_old[o] + (double)z[o];
+  };
+  const RowWalk rw = xcd_rows_mr<K>(n);
   double apr = 0.0, apq = 0.0;
-  for (int row = gw * R + grp; row < n; row += nw * R) {
-    const int e0 = rp[row], e1 = rp[row + 1];
-    double acc = 0.0;
-    for (int e = e0; e < e1; e++) {
-      const size_t o = (size_t)ci[e] * K + c;
-      const double pj = first ? (double)z[o] : beta * p_old[o] + (double)z[o];
-      acc += av[e] * pj;
-    }
+  for (int row = rw.first; row < rw.hi; row += rw.step) {
+    const int e0 = ld_stream(rp + row), e1 = ld_stream(rp + row + 1);
     const size_t o = (size_t)row * K + c;
-    const double pi = first ? (double)z[o] : beta * p_old[o] + (double)z[o];
+    const double pi = pval(row), ri = r[o];
+    const double acc = row_dot_a<K>(e0, e1, ci, av, pval);
     p_new[o] = pi;
     q[o] = acc;
-    apr += pi * r[o];
+    apr += pi * ri;
     apq += pi * acc;
   }
   double tot = 0.0;
-  if (column_grid_reduce<K>(apr, partials, &bs->ctr[CTR_SPMVP], tot) && threadIdx.x < K)
+  if (column_grid_reduce<K>(apr, partials, &bs->ctr[CTR_SPMVP], tot, bs->strict) &&
+      threadIdx.x < K)
     bs->pr[threadIdx.x] = tot;
   // second reduction on a disjoint partial region
-  if (column_grid_reduce<K>(apq, partials + (size_t)gridDim.x * K, &bs->ctr[CTR_GEN], tot) &&
+  if (column_grid_reduce<K>(apq, partials + (size_t)gridDim.x * K, &bs->ctr[CTR_GEN], tot,
+                            bs->strict) &&
       threadIdx.x < K) {
     const int cc = threadIdx.x;
     bs->pq[cc] = tot;
@@ -247,7 +366,7 @@ __global__ __launch_bounds__(256) void k_update_mr(int n, const double *__restri
     acc += ri * ri;
   }
   double tot = 0.0;
-  if (column_grid_reduce<K>(acc, partials, &bs->ctr[CTR_UPD], tot)) {
+  if (column_grid_reduce<K>(acc, partials, &bs->ctr[CTR_UPD], tot, bs->strict)) {
     if (threadIdx.x < K) {
       const int cc = threadIdx.x;
       if (!bs->done[cc]) {
@@ -271,22 +390,22 @@ __global__ __launch_bounds__(256) void k_spmv_res_mr(int n, const int *__restric
                                                      const double *__restrict__ bvec,
                                                      double *__restrict__ y, double *partials,
                                                      BatchScalars *bs) {
-  constexpr int R = 64 / K;
-  const int lane = threadIdx.x & 63, grp = lane / K, c = lane % K;
-  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int nw = gridDim.x * (blockDim.x >> 6);
+  const int c = (threadIdx.x & 63) % K;
+  const RowWalk rw = xcd_rows_mr<K>(n);
   double aqq = 0.0;
-  for (int row = gw * R + grp; row < n; row += nw * R) {
-    double acc = 0.0;
-    for (int e = rp[row]; e < rp[row + 1]; e++) acc += av[e] * x[(size_t)ci[e] * K + c];
+  for (int row = rw.first; row < rw.hi; row += rw.step) {
+    const int e0 = ld_stream(rp + row), e1 = ld_stream(rp + row + 1);
     const size_t o = (size_t)row * K + c;
-    if (bvec) acc -= bvec[o];
+    const double bi = bvec ? bvec[o] : 0.0;
+    double acc = row_dot_a<K>(e0, e1, ci, av, [&](int j) { return x[(size_t)j * K + c]; });
+    acc -= bi;
     y[o] = acc;
     aqq += acc * acc;
   }
   if (bvec) {
     double tot = 0.0;
-    if (column_grid_reduce<K>(aqq, partials, &bs->ctr[CTR_RES], tot) && threadIdx.x < K)
+    if (column_grid_reduce<K>(aqq, partials, &bs->ctr[CTR_RES], tot, bs->strict) &&
+        threadIdx.x < K)
       bs->relres[threadIdx.x] = sqrt(tot) / sqrt(bs->bb[threadIdx.x]);
   }
 }
