@@ -375,12 +375,20 @@ def run_c4a(args):
                 tb.append(sv.stats()["last_solve_ms"] / 1e3)
             true = [float(np.linalg.norm(Ap @ X[:, c] - B[:, c]) / np.linalg.norm(B[:, c]))
                     for c in range(k)]
+            # per-kernel device time of the batched loop (HIP events)
+            sv.set_option(rchol_amd.solver.OPT_PROFILE, 1)
+            sv.pcg_batch(B, args.tol, args.maxit)
+            stp = sv.stats()
+            sv.set_option(rchol_amd.solver.OPT_PROFILE, 0)
+            kms = {KERNEL_NAMES[i]: stp["kernel_ms"][i] / max(stp["kernel_iters"], 1)
+                   for i in range(4)}
             out[name] = {"single_rhs_s": float(np.mean(t1)), "single_iters": it1,
                          "batch_s": float(np.mean(tb)), "per_rhs_s": float(np.mean(tb)) / k,
                          "iters_max": int(it.max()), "iters_min": int(it.min()),
-                         "max_true_relres": max(true)}
+                         "max_true_relres": max(true), "batch_kernel_ms_per_iter": kms}
             log(f"c4a {name}: single {out[name]['single_rhs_s']*1e3:.1f} ms, batch of {k} "
-                f"{out[name]['batch_s']*1e3:.1f} ms")
+                f"{out[name]['batch_s']*1e3:.1f} ms; per iteration "
+                + ", ".join(f"{kk} {vv:.2f} ms" for kk, vv in kms.items()))
     per_rhs = out["mixed"]["per_rhs_s"]
     res = _base("PCG per-RHS time-to-1e-10, 32-RHS batch on the 256^3 var-coef problem",
                 per_rhs, "s", args.steps, 0, False,

@@ -70,3 +70,26 @@ def test_stream_cut_at_scale(system, waves):
     assert abs(i1 - i0) <= 1
     true = np.linalg.norm(Ap @ x1 - b) / np.linalg.norm(b)
     assert true <= 1e-9
+
+
+@pytest.mark.parametrize("k", [3, 8, 32])
+def test_batch_columns_match_single_solves(system, k):
+    """The K-column sweeps (their own task cut and wave lists) against the
+    single-right-hand-side path, column by column, at a size where every
+    level is cut for the full grid."""
+    Ap, Gs, b = system
+    print(f"case batch-vs-single k={k}", flush=True)
+    rng = np.random.default_rng(100 + k)
+    B = rng.uniform(-1.0, 1.0, (Ap.shape[0], k))
+    with rchol_amd.Solver(Ap, Gs) as s:
+        Z = s.precond_batch(B)
+        X, rr, it = s.pcg_batch(B, 1e-10, 500)
+        for c in range(k):
+            z1 = s.precond(B[:, c].copy())
+            assert np.linalg.norm(Z[:, c] - z1) / np.linalg.norm(z1) <= 1e-12
+            if c < 3:
+                x1, r1, i1 = s.pcg(B[:, c].copy(), 1e-10, 500)
+                assert abs(int(it[c]) - i1) <= 1
+                assert np.linalg.norm(X[:, c] - x1) / np.linalg.norm(x1) <= 1e-8
+            true = np.linalg.norm(Ap @ X[:, c] - B[:, c]) / np.linalg.norm(B[:, c])
+            assert true <= 1e-9
