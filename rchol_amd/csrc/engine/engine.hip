@@ -121,6 +121,30 @@ void build_slots(const std::vector<int> &lev, int nlev, HostSweep &hs, std::vect
   hs.level_ptr.assign(cnt.begin(), cnt.end());
 }
 
+// Inside every row, entries ordered by the level of the row they read,
+// latest first (stable insertion sort; rows are short).  The lane that polls
+// for a wave watches the first pending entry, which is then the value
+// produced last, so when it lands the rest of the row is usually there too.
+void order_entries_by_level(HostSweep &hs, const std::vector<int> &lev) {
+  const size_t n = hs.slot_row.size();
+  for (size_t s = 0; s < n; s++) {
+    const int e0 = hs.rp[s], e1 = hs.rp[s + 1];
+    for (int e = e0 + 1; e < e1; e++) {
+      const int j = hs.ci[e];
+      const double v = hs.cv[e];
+      const int lj = lev[j];
+      int k = e;
+      while (k > e0 && lev[hs.ci[k - 1]] < lj) {
+        hs.ci[k] = hs.ci[k - 1];
+        hs.cv[k] = hs.cv[k - 1];
+        k--;
+      }
+      hs.ci[k] = j;
+      hs.cv[k] = v;
+    }
+  }
+}
+
 // Cut a sweep's levels into a narrow head, a middle and a narrow tail
 // (levels of at most `width` rows run in one workgroup; width 0 = no split).
 void compute_split(const std::vector<int> &lp, int L, int width, int &h, int &t) {
@@ -1429,7 +1453,8 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
     }
     hb.diag[s] = diag[row];
   }
-
+  order_entries_by_level(hf, levf);
+  order_entries_by_level(hb, levb);
   return RCHOL_AMD_OK;
 }
 
