@@ -305,3 +305,22 @@ def test_stream_tasks_match_oracle(name, waves, precision):
         assert rel(z, d["z"]) <= 1e-5
         assert itr <= int(d["itr"]) + 5
     assert true <= 1e-9 and relres == pytest.approx(true, rel=1e-6)
+
+
+def test_option_values_are_validated():
+    A, Gs, d = load("lap3d_8_nd4")
+    print("case options", flush=True)
+    lib = rchol_amd._lib.engine()
+    with rchol_amd.Solver(A, Gs) as s:
+        h = s._h
+        for key, bad in ((rchol_amd.solver.OPT_TRSV_LANES, 3), (rchol_amd.solver.OPT_TRSV_LANES, 128),
+                         (rchol_amd.solver.OPT_SPMV_LANES, 5), (rchol_amd.solver.OPT_TASK_WAVES, -1),
+                         (rchol_amd.solver.OPT_SERIAL_WIDTH, -1), (99, 0)):
+            assert lib.rchol_amd_set_option(h, key, float(bad)) == -1
+        for key, good in ((rchol_amd.solver.OPT_TRSV_LANES, 64), (rchol_amd.solver.OPT_SPMV_LANES, 0),
+                          (rchol_amd.solver.OPT_TASK_WAVES, 0), (rchol_amd.solver.OPT_STRICT_SYNC, 1)):
+            assert lib.rchol_amd_set_option(h, key, float(good)) == 0
+        z = s.precond(d["b"])
+        x, relres, itr = s.pcg(d["b"], 1e-10, 5000)
+    assert rel(z, d["z"]) <= PRECOND_RTOL
+    assert abs(itr - int(d["itr"])) <= 1
