@@ -89,60 +89,6 @@ __device__ __forceinline__ T ld_stream(const T *p) {
 // flag is up every other wait returns at its next check.
 constexpr long long kWaitTicks = 1ll << 26;
 
-// Value of x[j] once its producer has published it.  Wave-cooperative: all
-// active lanes read their value once; while some still see the sentinel,
-// only the first such lane polls (agent-scope load, exponential back-off up
-// to ~2k clocks) and the others re-read once it has seen its value.  This
-// keeps the polling traffic of thousands of resident waves to one load per
-// waiting wave instead of one per waiting lane.  The error flag is read
-// only on the slow path, and a wait that outlives kWaitTicks raises it.
-template <typename T>
-__device__ __forceinline__ T wait_ready(const T *p, int *err, int coherent = 0) {
-  T v = coherent ? Sent<T>::load_agent(p) : *p;
-  bool pending = Sent<T>::is(v);
-  unsigned long long mask = __ballot(pending);
-  if (!mask) return v;
-  // a plain load may have hit a line cached before the value was published:
-  // re-read once, coherently, before anybody sleeps
-  if (pending && !coherent) {
-    v = Sent<T>::load_agent(p);
-    pending = Sent<T>::is(v);
-  }
-  mask = __ballot(pending);
-  if (!mask) return v;
-  const int lane = __lane_id();
-  const long long t0 = wall_clock64();
-  int backoff = 1;
-  while (mask) {
-    const int leader = __ffsll((long long)mask) - 1;
-    if (lane == leader) {
-      for (;;) {
-        for (int i = 0; i < backoff; i++) __builtin_amdgcn_s_sleep(2);
-        v = Sent<T>::load_agent(p);
-        if (!Sent<T>::is(v)) break;
-        if (backoff < 16) backoff <<= 1;
-        if (wall_clock64() - t0 > kWaitTicks ||
-            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-          atomicOr(err, 1);
-          v = T(0);
-          break;
-        }
-      }
-      pending = false;
-    } else if (pending) {
-      v = Sent<T>::load_agent(p);
-      pending = Sent<T>::is(v);
-    }
-    mask = __ballot(pending);
-    // a raised error releases every waiter
-    if (mask && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-      if (pending) v = T(0);
-      break;
-    }
-  }
-  return v;
-}
-
 // Back-off cap of the polling lane of await_batch, in s_sleep(1) units
 // (64 clocks each): bounds the extra delay between publication and
 // observation to ~kPollCap * 64 clocks.
@@ -150,6 +96,7 @@ __device__ __forceinline__ T wait_ready(const T *p, int *err, int coherent = 0) 
 #define RCHOL_AMD_POLL_CAP 8
 #endif
 constexpr int kPollCap = RCHOL_AMD_POLL_CAP;
+
 
 // Completes a round of U gathers per lane, xv[u] = x[j[u]] (j[u] < 0: an
 // unused slot), once some lane of the wave still holds a sentinel.  Called
