@@ -1130,12 +1130,24 @@ namespace {
 template <int L>
 void launch_spmv_p_seg_t(rchol_amd_ctx *c, SegLayout lay, const void *z, bool z32,
                          const double *p_old, double *p_new) {
+  if (L == 0) {  // stream form
+    if (z32)
+      hipLaunchKernelGGL((k_spmv_p_seg_s<float>), dim3(c->segs.nblocks), dim3(kBlock), 0,
+                         c->stream, lay, c->arp.p, c->aci.p, c->av.p, (const float *)z, p_old,
+                         p_new, c->q.p, (const double *)c->r.p, c->partials.p, c->segs.ss);
+    else
+      hipLaunchKernelGGL((k_spmv_p_seg_s<double>), dim3(c->segs.nblocks), dim3(kBlock), 0,
+                         c->stream, lay, c->arp.p, c->aci.p, c->av.p, (const double *)z, p_old,
+                         p_new, c->q.p, (const double *)c->r.p, c->partials.p, c->segs.ss);
+    return;
+  }
+  constexpr int LL = L ? L : 8;
   if (z32)
-    hipLaunchKernelGGL((k_spmv_p_seg<float, L>), dim3(c->segs.nblocks), dim3(kBlock), 0,
+    hipLaunchKernelGGL((k_spmv_p_seg<float, LL>), dim3(c->segs.nblocks), dim3(kBlock), 0,
                        c->stream, lay, c->arp.p, c->aci.p, c->av.p, (const float *)z, p_old, p_new,
                        c->q.p, (const double *)c->r.p, c->partials.p, c->segs.ss);
   else
-    hipLaunchKernelGGL((k_spmv_p_seg<double, L>), dim3(c->segs.nblocks), dim3(kBlock), 0,
+    hipLaunchKernelGGL((k_spmv_p_seg<double, LL>), dim3(c->segs.nblocks), dim3(kBlock), 0,
                        c->stream, lay, c->arp.p, c->aci.p, c->av.p, (const double *)z, p_old,
                        p_new, c->q.p, (const double *)c->r.p, c->partials.p, c->segs.ss);
 }
@@ -1165,6 +1177,7 @@ void rchol_amd_ctx::launch_iteration_seg(int parity, int slot) {
                        (const double *)z64.p, partials.p, segs.ss);
   const void *z = f32 ? (const void *)z32.p : (const void *)z64.p;
   switch (spmv_lanes) {
+    case 0: launch_spmv_p_seg_t<0>(this, lay, z, f32, p_old, p_new); break;
     case 4: launch_spmv_p_seg_t<4>(this, lay, z, f32, p_old, p_new); break;
     case 16: launch_spmv_p_seg_t<16>(this, lay, z, f32, p_old, p_new); break;
     case 32: launch_spmv_p_seg_t<32>(this, lay, z, f32, p_old, p_new); break;
@@ -1192,6 +1205,8 @@ int rchol_amd_ctx::pcg_segments(const double *d_b, double tol, int maxit, double
   segs.hss->tol = tol;
   segs.hss->maxit = maxit;
   segs.hss->nseg = segs.S;
+  segs.hss->strict = strict_sync;
+  HIP_TRY(hipMemcpyAsync(&segs.ss->strict, &segs.hss->strict, sizeof(int), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(&segs.ss->tol, &segs.hss->tol, sizeof(double), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(&segs.ss->maxit, &segs.hss->maxit, sizeof(int), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(&segs.ss->nseg, &segs.hss->nseg, sizeof(int), hipMemcpyHostToDevice, stream));
@@ -1257,6 +1272,7 @@ int rchol_amd_ctx::pcg_segments(const double *d_b, double tol, int maxit, double
   if (hsc->err)
     return fail(RCHOL_AMD_ESTALL, "triangular sweep wait exceeded its bound (err=%d)", hsc->err);
   switch (spmv_lanes) {
+    case 0: hipLaunchKernelGGL(k_spmv_res_seg_s, dim3(segs.nblocks), dim3(kBlock), 0, stream, lay, arp.p, aci.p, av.p, (const double *)x.p, d_b, tmp64a.p, partials.p, segs.ss); break;
     case 4: hipLaunchKernelGGL((k_spmv_res_seg<4>), dim3(segs.nblocks), dim3(kBlock), 0, stream, lay, arp.p, aci.p, av.p, (const double *)x.p, d_b, tmp64a.p, partials.p, segs.ss); break;
     case 16: hipLaunchKernelGGL((k_spmv_res_seg<16>), dim3(segs.nblocks), dim3(kBlock), 0, stream, lay, arp.p, aci.p, av.p, (const double *)x.p, d_b, tmp64a.p, partials.p, segs.ss); break;
     case 32: hipLaunchKernelGGL((k_spmv_res_seg<32>), dim3(segs.nblocks), dim3(kBlock), 0, stream, lay, arp.p, aci.p, av.p, (const double *)x.p, d_b, tmp64a.p, partials.p, segs.ss); break;

@@ -8,7 +8,7 @@
 // contention.  Every system keeps its own PCG scalars and stops on its own
 // test (pcg.cpp:82); a converged system is frozen.
 #pragma once
-#include "device.hpp"
+#include "kernels.hpp"
 
 namespace rchol_amd {
 
@@ -25,6 +25,7 @@ struct SegScalars {
   int all_done;
   int maxit;
   int err;
+  int strict;  // 1: partials published with release/acquire fences
   double tol;
 };
 
@@ -34,35 +35,53 @@ struct SegLayout {
   const int *seg, *r0, *r1, *sb;
 };
 
-// Sum NV values over the block, store the block's partials, and report
-// whether this block is the last of its system.  Block-uniform result.
+// Sum NV values over the block, publish the block's partials, and report
+// whether this block is the last of its system.  Block-uniform result.  The
+// handshake is grid_reduce's: value-returning atomic exchanges and a relaxed
+// counter (strict == 0; seg_total then reads with atomic loads), or plain
+// stores and an acquire-release counter (strict == 1).
 template <int NV>
 __device__ __forceinline__ bool seg_reduce(double (&v)[NV], double *partials, unsigned int *ctr,
-                                           int nblocks_in_seg) {
+                                           int nblocks_in_seg, int strict) {
   block_sum<NV>(v);
   __shared__ int is_last;
   if (threadIdx.x == 0) {
+    unsigned int prev;
+    if (strict) {
 #pragma unroll
-    for (int i = 0; i < NV; i++) partials[(size_t)blockIdx.x * NV + i] = v[i];
-    unsigned int prev =
-        __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      for (int i = 0; i < NV; i++) partials[(size_t)blockIdx.x * NV + i] = v[i];
+      prev = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      unsigned long long got = 0;
+#pragma unroll
+      for (int i = 0; i < NV; i++)
+        got ^= __hip_atomic_exchange(
+            (unsigned long long *)(partials + (size_t)blockIdx.x * NV + i),
+            (unsigned long long)__double_as_longlong(v[i]), __ATOMIC_RELAXED,
+            __HIP_MEMORY_SCOPE_AGENT);
+      asm volatile("" ::"v"(got) : "memory");
+      prev = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     is_last = (prev == (unsigned int)nblocks_in_seg - 1);
     if (is_last) __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   __syncthreads();
-  if (is_last) __atomic_thread_fence(__ATOMIC_ACQUIRE);
+  if (is_last && strict) __atomic_thread_fence(__ATOMIC_ACQUIRE);
   return is_last;
 }
 
 // Thread 0 of a system's last block: sum the system's partials in block order.
 template <int NV>
 __device__ __forceinline__ void seg_total(const double *partials, int b0, int b1,
-                                          double (&tot)[NV]) {
+                                          double (&tot)[NV], int strict) {
 #pragma unroll
   for (int i = 0; i < NV; i++) tot[i] = 0.0;
   for (int b = b0; b < b1; b++) {
 #pragma unroll
-    for (int i = 0; i < NV; i++) tot[i] += partials[(size_t)b * NV + i];
+    for (int i = 0; i < NV; i++) {
+      const double *slot = partials + (size_t)b * NV + i;
+      tot[i] += strict ? *slot : Sent<double>::load_agent(slot);
+    }
   }
 }
 
@@ -94,9 +113,9 @@ __global__ __launch_bounds__(256) void k_init_seg(SegLayout lay, const double *_
   }
   double v[1] = {acc};
   const int b0 = lay.sb[s], b1 = lay.sb[s + 1];
-  if (seg_reduce<1>(v, partials, &ss->ctr[0][s], b1 - b0) && threadIdx.x == 0) {
+  if (seg_reduce<1>(v, partials, &ss->ctr[0][s], b1 - b0, ss->strict) && threadIdx.x == 0) {
     double t[1];
-    seg_total<1>(partials, b0, b1, t);
+    seg_total<1>(partials, b0, b1, t, ss->strict);
     ss->bb[s] = t[0];
     ss->rr[s] = t[0];
     ss->k[s] = 0;
@@ -119,9 +138,9 @@ __global__ __launch_bounds__(256) void k_dot_seg(SegLayout lay, const double *__
     acc += r[i] * (double)z[i];
   double v[1] = {acc};
   const int b0 = lay.sb[s], b1 = lay.sb[s + 1];
-  if (seg_reduce<1>(v, partials, &ss->ctr[1][s], b1 - b0) && threadIdx.x == 0) {
+  if (seg_reduce<1>(v, partials, &ss->ctr[1][s], b1 - b0, ss->strict) && threadIdx.x == 0) {
     double t[1];
-    seg_total<1>(partials, b0, b1, t);
+    seg_total<1>(partials, b0, b1, t, ss->strict);
     ss->beta[s] = ss->k[s] > 0 ? t[0] / ss->rz_old[s] : 0.0;
     ss->rz_old[s] = t[0];
   }
@@ -164,9 +183,58 @@ __global__ __launch_bounds__(256) void k_spmv_p_seg(SegLayout lay, const int *__
   }
   double v[2] = {apr, apq};
   const int b0 = lay.sb[s], b1 = lay.sb[s + 1];
-  if (seg_reduce<2>(v, partials, &ss->ctr[2][s], b1 - b0) && threadIdx.x == 0) {
+  if (seg_reduce<2>(v, partials, &ss->ctr[2][s], b1 - b0, ss->strict) && threadIdx.x == 0) {
     double t[2];
-    seg_total<2>(partials, b0, b1, t);
+    seg_total<2>(partials, b0, b1, t, ss->strict);
+    ss->pr[s] = t[0];
+    ss->alpha[s] = t[0] / t[1];
+  }
+}
+
+// k_spmv_p_seg in stream form (RCHOL_AMD_OPT_SPMV_LANES = 0): the waves of
+// a block walk its row range 64 rows at a time (stream_rows).
+template <typename ZT>
+__global__ __launch_bounds__(256) void k_spmv_p_seg_s(SegLayout lay, const int *__restrict__ rp,
+                                                      const int *__restrict__ ci,
+                                                      const double *__restrict__ av,
+                                                      const ZT *__restrict__ z,
+                                                      const double *__restrict__ p_old,
+                                                      double *__restrict__ p_new,
+                                                      double *__restrict__ q,
+                                                      const double *__restrict__ r,
+                                                      double *partials, SegScalars *ss) {
+  if (ss->all_done) return;
+  __shared__ double sbuf[4][64 * kSpmvStream];
+  double *buf = sbuf[(threadIdx.x >> 6) & 3];
+  const int s = lay.seg[blockIdx.x];
+  const bool first = ss->k[s] == 0;
+  const double beta = ss->beta[s];
+  auto pval = [&](int j) -> double {
+    return first ? (double)z[j] : beta * p_old[j] + (double)z[j];
+  };
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  const int hi = lay.r1[blockIdx.x];
+  double apr = 0.0, apq = 0.0;
+  for (int r0 = lay.r0[blockIdx.x] + wv * 64; r0 < hi; r0 += nwv * 64) {
+    const int row = r0 + lane;
+    double pi = 0.0, ri = 0.0;
+    if (row < hi) {
+      pi = pval(row);
+      ri = r[row];
+    }
+    const double acc = stream_rows(r0, hi, rp, ci, av, pval, buf);
+    if (row < hi) {
+      p_new[row] = pi;
+      q[row] = acc;
+      apr += pi * ri;
+      apq += pi * acc;
+    }
+  }
+  double v[2] = {apr, apq};
+  const int b0 = lay.sb[s], b1 = lay.sb[s + 1];
+  if (seg_reduce<2>(v, partials, &ss->ctr[2][s], b1 - b0, ss->strict) && threadIdx.x == 0) {
+    double t[2];
+    seg_total<2>(partials, b0, b1, t, ss->strict);
     ss->pr[s] = t[0];
     ss->alpha[s] = t[0] / t[1];
   }
@@ -194,10 +262,10 @@ __global__ __launch_bounds__(256) void k_update_seg(SegLayout lay, const double 
   }
   double v[1] = {acc};
   const int b0 = lay.sb[s], b1 = lay.sb[s + 1];
-  if (seg_reduce<1>(v, partials, &ss->ctr[3][s], b1 - b0) && threadIdx.x == 0) {
+  if (seg_reduce<1>(v, partials, &ss->ctr[3][s], b1 - b0, ss->strict) && threadIdx.x == 0) {
     if (!ss->done[s]) {
       double t[1];
-      seg_total<1>(partials, b0, b1, t);
+      seg_total<1>(partials, b0, b1, t, ss->strict);
       const int k = ss->k[s] + 1;
       ss->k[s] = k;
       ss->rr[s] = t[0];
@@ -232,9 +300,42 @@ __global__ __launch_bounds__(256) void k_spmv_res_seg(SegLayout lay, const int *
   }
   double v[1] = {aqq};
   const int b0 = lay.sb[s], b1 = lay.sb[s + 1];
-  if (seg_reduce<1>(v, partials, &ss->ctr[4][s], b1 - b0) && threadIdx.x == 0) {
+  if (seg_reduce<1>(v, partials, &ss->ctr[4][s], b1 - b0, ss->strict) && threadIdx.x == 0) {
     double t[1];
-    seg_total<1>(partials, b0, b1, t);
+    seg_total<1>(partials, b0, b1, t, ss->strict);
+    ss->relres[s] = sqrt(t[0]) / sqrt(ss->bb[s]);
+  }
+}
+
+// k_spmv_res_seg in stream form.
+__global__ __launch_bounds__(256) void k_spmv_res_seg_s(SegLayout lay, const int *__restrict__ rp,
+                                                        const int *__restrict__ ci,
+                                                        const double *__restrict__ av,
+                                                        const double *__restrict__ x,
+                                                        const double *__restrict__ bvec,
+                                                        double *__restrict__ y, double *partials,
+                                                        SegScalars *ss) {
+  __shared__ double sbuf[4][64 * kSpmvStream];
+  double *buf = sbuf[(threadIdx.x >> 6) & 3];
+  const int s = lay.seg[blockIdx.x];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  const int hi = lay.r1[blockIdx.x];
+  double aqq = 0.0;
+  for (int r0 = lay.r0[blockIdx.x] + wv * 64; r0 < hi; r0 += nwv * 64) {
+    const int row = r0 + lane;
+    const double bi = row < hi ? bvec[row] : 0.0;
+    double acc = stream_rows(r0, hi, rp, ci, av, [&](int j) { return x[j]; }, buf);
+    if (row < hi) {
+      acc -= bi;
+      y[row] = acc;
+      aqq += acc * acc;
+    }
+  }
+  double v[1] = {aqq};
+  const int b0 = lay.sb[s], b1 = lay.sb[s + 1];
+  if (seg_reduce<1>(v, partials, &ss->ctr[4][s], b1 - b0, ss->strict) && threadIdx.x == 0) {
+    double t[1];
+    seg_total<1>(partials, b0, b1, t, ss->strict);
     ss->relres[s] = sqrt(t[0]) / sqrt(ss->bb[s]);
   }
 }

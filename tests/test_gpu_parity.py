@@ -220,9 +220,11 @@ def test_fp32_factor_with_residual_replacement(period):
 
 
 @pytest.mark.parametrize("precision", [rchol_amd.FP64, rchol_amd.FP32])
-def test_uneven_batch_matches_per_system_oracle(precision):
+@pytest.mark.parametrize("lanes", [0, 8])
+@pytest.mark.parametrize("strict", [0, 1])
+def test_uneven_batch_matches_per_system_oracle(precision, lanes, strict):
     from rchol_amd import factor as F
-    print(f"case segments prec={precision}", flush=True)
+    print(f"case segments prec={precision} spmv_lanes={lanes} strict={strict}", flush=True)
     rng = np.random.default_rng(48)
     systems = []
     for s, side in enumerate([6, 13, 9, 20, 7]):
@@ -233,7 +235,8 @@ def test_uneven_batch_matches_per_system_oracle(precision):
         systems.append((A[P][:, P].tocsr(), Gs))
     Au, Gu, off = rchol_amd.stack_systems(systems)
     b = rng.uniform(-1, 1, Au.shape[0])
-    with rchol_amd.Solver(Au, Gu, precision=precision) as s:
+    with rchol_amd.Solver(Au, Gu, precision=precision, spmv_lanes=lanes) as s:
+        s.set_option(rchol_amd.solver.OPT_STRICT_SYNC, strict)
         s.set_segments(off)
         x, rr, it = s.pcg_segments(b, 1e-10, 2000)
         x2, rr2, it2 = s.pcg_segments(b, 1e-10, 2000)  # graph replay, same result
