@@ -468,21 +468,28 @@ def run_c5(args):
         b0 = rng.uniform(-1, 1, n) if not grads else b0
         xs = np.random.default_rng(2).uniform(-1, 1, n)
         theta = torch.tensor(theta0, device=dev, requires_grad=True)
-        w = torch.nn.functional.softplus(theta)
-        prob.solver(w.detach().cpu().numpy())  # factor outside the timed region
+        bt = torch.tensor(b0, device=dev)
+        xt = torch.tensor(xs, device=dev)
+
+        def step():
+            theta.grad = None
+            w = torch.nn.functional.softplus(theta)
+            x = AG.solve(w, bt, prob)
+            ((x - xt) ** 2).sum().backward()
+
+        step()  # factor, upload and graph capture outside the timed region
         torch.cuda.synchronize()
         t = time.perf_counter()
-        x = AG.solve(w, torch.tensor(b0, device=dev), prob)
-        loss = ((x - torch.tensor(xs, device=dev)) ** 2).sum()
-        loss.backward()
+        for _ in range(max(args.steps, 1)):
+            step()
         torch.cuda.synchronize()
-        times[name] = time.perf_counter() - t
+        times[name] = (time.perf_counter() - t) / max(args.steps, 1)
         grads[name] = theta.grad.cpu().numpy()
         log(f"c5 {name}: forward {prob.last['forward']}, adjoint {prob.last['adjoint']}, "
             f"{times[name]:.3f}s")
     g, gr = grads["mixed"], grads["fp64"]
     res = _base("differentiable solve: forward + adjoint time, 100^3 graph Laplacian",
-                times["mixed"], "s", 1, 0, False,
+                times["mixed"], "s", max(args.steps, 1), 1, False,
                 {"workload": f"{side}^3 6-neighbour graph Laplacian, w = softplus(theta), "
                              "+1e-2 I, loss ||x - x*||^2, mixed forward/adjoint PCG",
                  "n": n, "edges": int(g.size)}, "f64 + f32 factor")

@@ -40,6 +40,7 @@ class GridLaplacian:
         self.v = torch.as_tensor(v, dtype=torch.int64, device=dev)
         self._key = None
         self._solver = None
+        self._w = None  # device copy of the weights the cached factor belongs to
         self.last = {}
 
     def matrix(self, w):
@@ -64,6 +65,17 @@ class GridLaplacian:
         return self._solver
 
 
+    def solver_for(self, w):
+        """Solver for the weights of a device tensor: the cached factor is
+        reused when w equals (on the device) the weights it was built for."""
+        if self._solver is not None and self._w is not None and self._w.shape == w.shape \
+                and torch.equal(self._w, w):
+            return self._solver
+        s = self.solver(w.detach().cpu().numpy())
+        self._w = w.detach().clone()
+        return s
+
+
 class RcholSolve(torch.autograd.Function):
     """x = A(w)^-1 b with hand-written adjoint (see module docstring)."""
 
@@ -73,7 +85,7 @@ class RcholSolve(torch.autograd.Function):
             raise TypeError("w and b must be float64")
         if not (w.is_cuda and b.is_cuda):
             raise ValueError("w and b must live on the GPU")
-        s = problem.solver(w.detach().cpu().numpy())
+        s = problem.solver_for(w.detach())
         b = b.contiguous()
         x = torch.empty_like(b)
         torch.cuda.current_stream(b.device).synchronize()
