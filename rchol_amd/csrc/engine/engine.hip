@@ -46,6 +46,24 @@ int fail(int code, const char *fmt, ...) {
   } while (0)
 
 constexpr int kTaskRows = 64;     // slots per triangular-sweep task
+// Grid caps, in blocks per CU, of the SpMV and of the streaming vector
+// kernels (measured at 128^3 and 256^3): a smaller SpMV grid keeps the rows
+// an XCD works on at once -- and the vector lines they gather -- within its
+// L2 (256^3: 0.86 ms at 8 per CU, 0.72 ms at 2, 1.02 ms at 1); the
+// streaming kernels reach the copy rate with 4.
+#ifndef RCHOL_AMD_SPMV_PER_CU
+#define RCHOL_AMD_SPMV_PER_CU 2
+#endif
+constexpr int kSpmvBlocksPerCu = RCHOL_AMD_SPMV_PER_CU;
+// The K-column SpMV gathers K values per entry and needs more rows in flight.
+#ifndef RCHOL_AMD_SPMV_MR_PER_CU
+#define RCHOL_AMD_SPMV_MR_PER_CU 8
+#endif
+constexpr int kSpmvMrBlocksPerCu = RCHOL_AMD_SPMV_MR_PER_CU;
+#ifndef RCHOL_AMD_VEC_PER_CU
+#define RCHOL_AMD_VEC_PER_CU 4
+#endif
+constexpr int kVecBlocksPerCu = RCHOL_AMD_VEC_PER_CU;
 constexpr int kBlock = 256;       // threads per block for every kernel
 constexpr size_t kIdxMax = 0x7FFFFFFFu;
 
@@ -122,9 +140,12 @@ void build_slots(const std::vector<int> &lev, int nlev, HostSweep &hs, std::vect
 }
 
 // Inside every row, entries ordered by the level of the row they read,
-// latest first (stable insertion sort; rows are short).  The lane that polls
-// for a wave watches the first pending entry, which is then the value
-// produced last, so when it lands the rest of the row is usually there too.
+// latest last (stable insertion sort; rows are short).  Single-column sweeps
+// read a row from its end, so the lane that polls for a wave watches the
+// value produced last and, when it lands, the rest of the row is usually
+// there too.  Multi-column sweeps read from the start, several rounds per
+// row: the rounds over long-finished entries run while the latest are
+// still in flight.
 void order_entries_by_level(HostSweep &hs, const std::vector<int> &lev) {
   const size_t n = hs.slot_row.size();
   for (size_t s = 0; s < n; s++) {
@@ -134,7 +155,7 @@ void order_entries_by_level(HostSweep &hs, const std::vector<int> &lev) {
       const double v = hs.cv[e];
       const int lj = lev[j];
       int k = e;
-      while (k > e0 && lev[hs.ci[k - 1]] < lj) {
+      while (k > e0 && lev[hs.ci[k - 1]] > lj) {
         hs.ci[k] = hs.ci[k - 1];
         hs.cv[k] = hs.cv[k - 1];
         k--;
@@ -432,12 +453,12 @@ struct rchol_amd_ctx {
 
   int vec_grid() const {
     size_t g = (n + kBlock - 1) / kBlock;
-    return (int)std::max<size_t>(1, std::min<size_t>(g, (size_t)ncu * 8));
+    return (int)std::max<size_t>(1, std::min<size_t>(g, (size_t)ncu * kVecBlocksPerCu));
   }
   int spmv_grid() const {
     size_t rows_per_block = (kBlock / 64) * (spmv_lanes ? 64 / spmv_lanes : 64);
     size_t g = (n + rows_per_block - 1) / rows_per_block;
-    g = std::max<size_t>(1, std::min<size_t>(g, (size_t)ncu * 8));
+    g = std::max<size_t>(1, std::min<size_t>(g, (size_t)ncu * kSpmvBlocksPerCu));
     if (g > 8) g = (g + 7) / 8 * 8;  // whole XCD rounds for the row walk of the stream form
     return (int)g;
   }
@@ -939,7 +960,7 @@ void rchol_amd_ctx::launch_iteration_mr(int parity, int slot) {
   launch_sweeps_mr<K>(batch.r.p, true, &batch.bs->all_done, ev);
   const int rows_per_block = (kBlock / 64) * (64 / K);
   int sg = (int)std::max<size_t>(
-      1, std::min<size_t>((n + rows_per_block - 1) / rows_per_block, (size_t)ncu * 8));
+      1, std::min<size_t>((n + rows_per_block - 1) / rows_per_block, (size_t)ncu * kSpmvMrBlocksPerCu));
   if (sg > 8) sg = (sg + 7) / 8 * 8;  // whole XCD rounds for the row walk
   const int vg = (int)std::max<size_t>(
       1, std::min<size_t>((n * K + kBlock - 1) / kBlock, (size_t)ncu * 8));
@@ -1043,7 +1064,7 @@ int rchol_amd_ctx::pcg_batch(const double *d_b, int ncols, double tol, int maxit
     return fail(RCHOL_AMD_ESTALL, "triangular sweep wait exceeded its bound (err=%d)", batch.hbs->err);
   const int rows_per_block = (kBlock / 64) * (64 / K);
   int sg = (int)std::max<size_t>(
-      1, std::min<size_t>((n + rows_per_block - 1) / rows_per_block, (size_t)ncu * 8));
+      1, std::min<size_t>((n + rows_per_block - 1) / rows_per_block, (size_t)ncu * kSpmvMrBlocksPerCu));
   if (sg > 8) sg = (sg + 7) / 8 * 8;  // whole XCD rounds for the row walk
   hipLaunchKernelGGL((k_spmv_res_mr<K>), dim3(sg), dim3(kBlock), 0, stream, (int)n, arp.p, aci.p,
                      av.p, (const double *)batch.x.p, d_b, batch.res.p, batch.partials.p, batch.bs);

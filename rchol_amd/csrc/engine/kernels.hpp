@@ -44,23 +44,25 @@ namespace rchol_amd {
 #endif
 constexpr int kGather = RCHOL_AMD_GATHER;  // gathers in flight per lane and round
 
-// Sum of val[e] * x[ci[e]] over the entries e0 + sub + k L < e1 of one row,
-// waiting for unpublished x.  Wave-converged call (every lane of the wave,
-// e0 == e1 for idle groups).
+// Sum of val[e] * x[ci[e]] over the entries e1 - 1 - sub - k L >= e0 of one
+// row, waiting for unpublished x.  The row is read from its end: its entries
+// are stored latest dependency last, so slot 0 of the first round -- the one
+// await_batch polls -- holds the value produced last.  Wave-converged call
+// (every lane of the wave, e0 == e1 for idle groups).
 template <typename VT, typename XT, int L, bool WAIT>
 __device__ __forceinline__ XT row_dot(int e0, int e1, int sub, const int *__restrict__ ci,
                                       const VT *__restrict__ cv, const XT *x, int *err,
                                       int coherent) {
   XT acc = 0;
-  for (int c = e0 + sub; c < e1; c += kGather * L) {
+  for (int c = e1 - 1 - sub; c >= e0; c -= kGather * L) {
     int j[kGather];
     VT v[kGather];
     XT xv[kGather];
 #pragma unroll
     for (int u = 0; u < kGather; u++) {
-      const int e = c + u * L;
-      j[u] = e < e1 ? ld_stream(ci + e) : -1;
-      v[u] = e < e1 ? ld_stream(cv + e) : VT(0);
+      const int e = c - u * L;
+      j[u] = e >= e0 ? ld_stream(ci + e) : -1;
+      v[u] = e >= e0 ? ld_stream(cv + e) : VT(0);
     }
 #pragma unroll
     for (int u = 0; u < kGather; u++) {
