@@ -73,3 +73,31 @@ def test_schedule_rejects_invalid_factors():
         rchol_amd.analyze((cp, np.array([0, 1, 1]), np.array([0.0, 1.0, 1.0])))  # zero pivot
     with pytest.raises(RuntimeError, match="error -1"):
         rchol_amd.analyze((cp, np.array([0, 0, 1]), np.ones(3)))  # two diagonals
+
+
+def test_threaded_analysis_matches_one_thread(monkeypatch):
+    """At 2^18 rows the analysis splits columns and slots over host threads;
+    its schedule must not depend on the thread count, and a malformed
+    factor must be reported at its lowest bad column, as one thread would."""
+    A = F.varcoef_3d(64, 64, 64, 1.5, 1e-3, 9)
+    perm, blocks = F.nd_grid(64, 64, 64, 5)
+    Gs, _, _ = F.factor(A, perm, blocks, "mis", 3)
+    monkeypatch.setenv("OMP_NUM_THREADS", "1")
+    one = rchol_amd.analyze(Gs, 0)
+    monkeypatch.setenv("OMP_NUM_THREADS", "8")
+    many = rchol_amd.analyze(Gs, 0)
+    for key in ("fwd", "bwd"):
+        assert one[key]["nlevels"] == many[key]["nlevels"]
+        assert np.array_equal(one[key]["level_ptr"], many[key]["level_ptr"])
+        assert np.array_equal(one[key]["slot_row"], many[key]["slot_row"])
+    cp, ri, gv = (np.array(a) for a in Gs)
+    n = cp.size - 1
+    bad = gv.copy()
+    for c in (n - 5, n // 3, n // 2):   # zero pivots; the lowest must be named
+        bad[int(cp[c])] = 0.0
+    with pytest.raises(RuntimeError, match=rf"G\({n // 3},{n // 3}\) is zero"):
+        rchol_amd.analyze((cp, ri, bad), 0)
+    cp2 = cp.copy()
+    cp2[n // 4 + 1] = cp2[n // 4] - 1        # a decreasing column pointer
+    with pytest.raises(RuntimeError, match=rf"not monotone at {n // 4}"):
+        rchol_amd.analyze((cp2, ri, gv), 0)
