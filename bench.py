@@ -116,6 +116,9 @@ def run_c3(args):
     st64 = s64.stats()
     log(f"sweep levels fwd={st64['fwd_levels']} bwd={st64['bwd_levels']} "
         f"waves={st64['trsv_waves']}")
+    setup = {name: dict(zip(("upload_a", "analysis", "schedule", "copies"), st["setup_ms"]))
+             for name, st in (("fp64", st64), ("mixed", s32.stats()))}
+    log("setup ms (mixed): " + ", ".join(f"{k} {v:.0f}" for k, v in setup["mixed"].items()))
 
     def run(solver, k):
         times, dev = [], []
@@ -228,10 +231,11 @@ def run_c3(args):
         },
         "mixed": {"time_s": solve32, "host_time_s": float(np.mean(t32)), "iters": it32,
                   "relres": rr32, "relres_fp64_verified": r_true32,
-                  "hbm_gbs": b32 * it32 / solve32 / 1e9, "kernels": prof["mixed"]},
+                  "hbm_gbs": b32 * it32 / solve32 / 1e9, "kernels": prof["mixed"],
+                  "setup_ms": setup["mixed"]},
         "fp64": {"time_s": solve64, "host_time_s": float(np.mean(t64)), "iters": it64,
                  "relres": rr64, "hbm_gbs": b64 * it64 / solve64 / 1e9,
-                 "kernels": prof["fp64"]},
+                 "kernels": prof["fp64"], "setup_ms": setup["fp64"]},
         "speedup_mixed_vs_fp64": solve64 / solve32,
         "error": err,
         "roofline": roofline,

@@ -76,6 +76,12 @@ typedef struct rchol_amd_stats {
    * SpMV, [3] x/r update -- and the iterations they cover. */
   double kernel_ms[4];
   int kernel_iters;
+  /* Host wall time of the last uploads: [0] rchol_amd_upload_A (checks,
+   * index narrowing, copies), and of rchol_amd_upload_G [1] the factor
+   * analysis (validation, levels, slot-ordered CSR of both sweeps), [2] the
+   * sweep schedule (task cut, per-wave lists), [3] the FP32 conversion (FP32
+   * factor) and the copies to the device. */
+  double setup_ms[4];
 } rchol_amd_stats;
 
 /* Level schedule of the two triangular sweeps (host analysis only). */

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -149,6 +150,13 @@ int host_threads() {
     if (v > 0) t = std::min(t, v);
   }
   return std::max(1, std::min(t, 32));
+}
+
+// Host wall clock in milliseconds (setup timing).
+double host_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
 }
 
 // f(lo, hi) over [0, n) in contiguous chunks, one per thread (chunks of at
@@ -1448,15 +1456,27 @@ int rchol_amd_upload_A(rchol_amd_ctx *c, size_t n, const size_t *rowPtr, const s
   const size_t nnz = rowPtr[n];
   if (rowPtr[0] != 0) return fail(RCHOL_AMD_EINVAL, "rowPtr[0] must be 0");
   if (nnz > kIdxMax) return fail(RCHOL_AMD_ERANGE, "nnz(A)=%zu exceeds the int32 index range", nnz);
-  std::vector<int> rp(n + 1), ci(nnz);
+  const double t0 = host_ms();
+  std::vector<int> rp(n + 1);
+  std::vector<int, NoInit<int>> ci(nnz);
   for (size_t i = 0; i <= n; i++) {
     if (i && rowPtr[i] < rowPtr[i - 1]) return fail(RCHOL_AMD_EINVAL, "rowPtr not monotone at %zu", i);
     rp[i] = (int)rowPtr[i];
   }
-  for (size_t e = 0; e < nnz; e++) {
-    if (colIdx[e] >= n) return fail(RCHOL_AMD_EINVAL, "colIdx[%zu]=%zu out of range", e, colIdx[e]);
-    ci[e] = (int)colIdx[e];
-  }
+  size_t bad = nnz;  // lowest out-of-range column index
+  par_for(nnz, [&](size_t e0, size_t e1) {
+    for (size_t e = e0; e < e1; e++) {
+      if (colIdx[e] >= n) {
+        size_t cur = __atomic_load_n(&bad, __ATOMIC_RELAXED);
+        while (e < cur &&
+               !__atomic_compare_exchange_n(&bad, &cur, e, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+        }
+        return;
+      }
+      ci[e] = (int)colIdx[e];
+    }
+  });
+  if (bad < nnz) return fail(RCHOL_AMD_EINVAL, "colIdx[%zu]=%zu out of range", bad, colIdx[bad]);
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->drop_graph();
@@ -1469,6 +1489,7 @@ int rchol_amd_upload_A(rchol_amd_ctx *c, size_t n, const size_t *rowPtr, const s
   c->hasA = true;
   c->stats.n = n;
   c->stats.nnz_a = nnz;
+  c->stats.setup_ms[0] = host_ms() - t0;
   return RCHOL_AMD_OK;
 }
 
@@ -1638,8 +1659,10 @@ int rchol_amd_upload_G(rchol_amd_ctx *c, size_t n, const size_t *colPtr, const s
   if (nnz > kIdxMax) return fail(RCHOL_AMD_ERANGE, "nnz(G)=%zu exceeds the int32 index range", nnz);
   HostSweep hf, hb;
   int nlf = 0, nlb = 0;
+  const double t0 = host_ms();
   int rc0 = analyze_factor(n, colPtr, rowIdx, val, hf, hb, nlf, nlb);
   if (rc0) return rc0;
+  const double t1 = host_ms();
   const size_t noff = nnz - n;
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1683,9 +1706,13 @@ int rchol_amd_upload_G(rchol_amd_ctx *c, size_t n, const size_t *colPtr, const s
   c->nnzG = nnz;
   c->gprec = precision;
   c->g_gen++;
+  const double t2 = host_ms();
   rc = c->size_trsv_grid();
   if (rc) return rc;
   c->hasG = true;
+  c->stats.setup_ms[1] = t1 - t0;
+  c->stats.setup_ms[2] = host_ms() - t2;
+  c->stats.setup_ms[3] = t2 - t1;
   c->stats.nnz_g = nnz;
   c->stats.fwd_levels = nlf;
   c->stats.bwd_levels = nlb;

@@ -41,11 +41,13 @@ class Stats(ctypes.Structure):
         ("last_relres", ctypes.c_double),
         ("kernel_ms", ctypes.c_double * 4),
         ("kernel_iters", ctypes.c_int),
+        ("setup_ms", ctypes.c_double * 4),
     ]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
         d["kernel_ms"] = list(self.kernel_ms)
+        d["setup_ms"] = list(self.setup_ms)
         return d
 
 
