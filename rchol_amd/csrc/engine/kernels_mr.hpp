@@ -297,8 +297,10 @@ __device__ __forceinline__ double row_dot_a(int e0, int e1, const int *__restric
 
 // p_new = z + beta_c p_old (p = z at k_c == 0), q = A p_new, per-column
 // p.r and p.q -> alpha_c.
+// Occupancy 8 (<= 64 VGPRs): the grid fills 8 workgroups per CU, and at 7
+// resident the eighth would run as a second wave of workgroups.
 template <typename ZT, int K>
-__global__ __launch_bounds__(256) void k_spmv_p_mr(int n, const int *__restrict__ rp,
+__global__ __launch_bounds__(256, 8) void k_spmv_p_mr(int n, const int *__restrict__ rp,
                                                    const int *__restrict__ ci,
                                                    const double *__restrict__ av,
                                                    const ZT *__restrict__ z,

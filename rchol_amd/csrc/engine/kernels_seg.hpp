@@ -193,8 +193,10 @@ __global__ __launch_bounds__(256) void k_spmv_p_seg(SegLayout lay, const int *__
 
 // k_spmv_p_seg in stream form (RCHOL_AMD_OPT_SPMV_LANES = 0): the waves of
 // a block walk its row range 64 rows at a time (stream_rows).
+// Occupancy 8 (<= 64 VGPRs): the grid fills 8 workgroups per CU, and at 7
+// resident the eighth would run as a second wave of workgroups.
 template <typename ZT>
-__global__ __launch_bounds__(256) void k_spmv_p_seg_s(SegLayout lay, const int *__restrict__ rp,
+__global__ __launch_bounds__(256, 8) void k_spmv_p_seg_s(SegLayout lay, const int *__restrict__ rp,
                                                       const int *__restrict__ ci,
                                                       const double *__restrict__ av,
                                                       const ZT *__restrict__ z,
