@@ -1,0 +1,121 @@
+"""The drop-in surfaces above the C ABI, on the GPU.
+
+- Python: ex_laplace.py's call sequence -- G, p, _ = rchol(A, ...)
+  (python/rchol/rchol.py:8-45), then x = pcg(A[p][:, p], b[p], tol, maxit,
+  G, G.T) (python/rchol/util.py:32-61) and x_orig[p] = x
+  (python/rchol/ex_laplace.py:14-31) -- checked against the oracle on the
+  same inputs and against the unpermuted system.
+- C++: the pcg(A, b, tol, maxit, G, x, relres, itr) constructor of
+  c++/util/pcg.hpp:15-16 (include/rchol_amd_pcg.hpp), through the example
+  driver that mirrors c++/ex_laplace.cpp, against the Python path on the
+  same factor.
+- The device-pointer entry points (torch tensors) against their host twins:
+  the same kernels run, so the results are bit-identical; a device batch
+  that is not 4, 8, 16 or 32 columns wide is refused.
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import rchol_amd
+from rchol_amd import factor as F
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if rchol_amd.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+def test_python_dropin_follows_ex_laplace():
+    n = 12
+    A = F.laplace_3d(n)
+    b = np.random.default_rng(3).uniform(-1.0, 1.0, A.shape[0])
+    G, perm, part = F.rchol(A, 4, grid=(n, n, n))
+    assert part[-1] == A.shape[0] + 1
+    p = perm.astype(np.int64)
+    Ap = A[p][:, p].tocsr()
+    print("case python_dropin", flush=True)
+    x, relres, niters = rchol_amd.pcg(Ap, b[p], 1e-10, 500, G, G.T.tocsr())
+    xo, ro, io = O.pcg(Ap, b[p], 1e-10, 500, rchol_amd.stored_from_lower(G))
+    assert abs(niters - io) <= 1
+    assert rel(x, xo) <= 1e-9
+    x_orig = np.empty_like(x)
+    x_orig[p] = x
+    true = np.linalg.norm(A @ x_orig - b) / np.linalg.norm(b)
+    assert relres == pytest.approx(true, rel=1e-6, abs=1e-14)
+    assert true <= 2e-10
+
+
+def test_cpp_pcg_shim_example():
+    exe = os.path.join(ROOT, "rchol_amd", "csrc", "bin", "ex_laplace")
+    assert os.path.exists(exe), "the C++ example is built by make -C rchol_amd/csrc all"
+    side, leaves, seed = 16, 4, 1
+    print("case cpp_shim", flush=True)
+    out = subprocess.run([exe, "-n", str(side), "-t", str(leaves), "-tol", "1e-10",
+                          "-maxit", "500", "-seed", str(seed)],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    fill = float(re.search(r"Fill-in ratio: (\S+)", out.stdout).group(1))
+    itr = int(re.search(r"# CG iterations: (\d+)", out.stdout).group(1))
+    relres = float(re.search(r"Relative residual: (\S+)", out.stdout).group(1))
+    # the same factor from the Python side: static order, ND with 4 leaves
+    A = F.laplace_3d(side)
+    perm, blocks = F.nd_grid(side, side, side, 3)
+    Gs, po, _ = F.factor(A, perm, blocks, "static", seed)
+    Ap = F.permute(A, po)
+    assert fill == pytest.approx(2.0 * Gs[0][-1] / Ap.nnz, rel=2e-3)
+    b = np.random.default_rng(5).uniform(-1.0, 1.0, A.shape[0])
+    with rchol_amd.Solver(Ap, Gs) as s:
+        _, rr_py, it_py = s.pcg(b, 1e-10, 500)
+    # a different right-hand side (the driver draws b with std::mt19937_64)
+    assert abs(itr - it_py) <= 3
+    assert 0.0 < relres <= 2e-10 and rr_py <= 2e-10
+
+
+def test_device_entry_points_match_host_ones():
+    import torch
+
+    d = np.load(os.path.join(GOLD, "lap3d_8_nd4.npz"))
+    n = d["a_indptr"].size - 1
+    A = sp.csr_matrix((d["a_data"], d["a_indices"], d["a_indptr"]), shape=(n, n))
+    Gs = (d["g_colptr"], d["g_rowidx"], d["g_val"])
+    b = d["b"]
+    print("case device_entry_points", flush=True)
+    for prec in (rchol_amd.FP64, rchol_amd.FP32):
+        with rchol_amd.Solver(A, Gs, prec) as s:
+            xh, rh, ih = s.pcg(b, 1e-10, 500)
+            bt = torch.from_numpy(b).cuda()
+            xt = torch.zeros_like(bt)
+            torch.cuda.synchronize()
+            rd, idv = s.pcg_device(bt.data_ptr(), xt.data_ptr(), 1e-10, 500)
+            assert idv == ih and rd == rh
+            assert np.array_equal(xt.cpu().numpy(), xh)
+            zt = torch.empty_like(bt)
+            s.precond_device(bt.data_ptr(), zt.data_ptr())
+            assert np.array_equal(zt.cpu().numpy(), s.precond(b))
+            # device batches take 4, 8, 16 or 32 interleaved columns
+            B = np.ascontiguousarray(np.stack([b, 2.0 * b, -b[::-1], b * b], axis=1))
+            Xh, rrh, ith = s.pcg_batch(B, 1e-10, 500)
+            Bt = torch.from_numpy(B).cuda()
+            Xt = torch.zeros_like(Bt)
+            torch.cuda.synchronize()
+            rrd, itd = s.pcg_batch_device(4, Bt.data_ptr(), Xt.data_ptr(), 1e-10, 500)
+            assert np.array_equal(itd, ith) and np.array_equal(rrd, rrh)
+            assert np.array_equal(Xt.cpu().numpy(), Xh)
+            with pytest.raises(RuntimeError, match="4, 8, 16 or 32 columns"):
+                s.pcg_batch_device(3, Bt.data_ptr(), Xt.data_ptr(), 1e-10, 500)
