@@ -108,6 +108,14 @@ int rchol_amd_destroy(rchol_amd_ctx *ctx);
 int rchol_amd_upload_A(rchol_amd_ctx *ctx, size_t n, const size_t *rowPtr,
                        const size_t *colIdx, const double *val);
 
+/* New values for the resident A, whose sparsity (rowPtr, colIdx) is
+ * unchanged: one copy of val[nnz], no index work.  For a sequence of
+ * matrices on one pattern, as python/rchol/ex_reuse_partition.py:28-42
+ * factors with a reused ordering.  No reference counterpart (the
+ * reference rebuilds its handles in every pcg() call, pcg.cpp:22-27);
+ * nnz must equal the resident matrix's. */
+int rchol_amd_update_A_values(rchol_amd_ctx *ctx, size_t nnz, const double *val);
+
 /* Device residency of the factor, from the arrays rchol() stores
  * (rchol_lap.cpp:358-387): colPtr[n+1], rowIdx[nnz], val[nnz].  Builds the
  * forward (G) and backward (G^T) sweeps with their dependency-level

@@ -33,6 +33,7 @@ ENGINE_SYMBOLS = {
     "rchol_amd_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     "rchol_amd_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "rchol_amd_upload_A": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, u64p, u64p, f64p]),
+    "rchol_amd_update_A_values": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, f64p]),
     "rchol_amd_upload_G": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, u64p, u64p, f64p,
                                           ctypes.c_int]),
     "rchol_amd_precond": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p]),

@@ -1493,6 +1493,20 @@ int rchol_amd_upload_A(rchol_amd_ctx *c, size_t n, const size_t *rowPtr, const s
   return RCHOL_AMD_OK;
 }
 
+int rchol_amd_update_A_values(rchol_amd_ctx *c, size_t nnz, const double *val) {
+  if (!c || (!val && nnz)) return fail(RCHOL_AMD_EINVAL, "null argument");
+  if (!c->hasA) return fail(RCHOL_AMD_ESTATE, "no matrix is resident");
+  if (nnz != c->nnzA)
+    return fail(RCHOL_AMD_EINVAL, "nnz=%zu differs from the resident matrix's %zu", nnz, c->nnzA);
+  const double t0 = host_ms();
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  // the values change in place: captured graphs keep their pointers
+  HIP_TRY(hipMemcpy(c->av.p, val, nnz * sizeof(double), hipMemcpyHostToDevice));
+  c->stats.setup_ms[0] = host_ms() - t0;
+  return RCHOL_AMD_OK;
+}
+
 // Host analysis of the stored factor: validation (lower triangle, one
 // nonzero diagonal per column), dependency levels of both sweeps and their
 // slot-ordered CSR.  Forward: row i of G waits for every column j < i with

@@ -5,6 +5,7 @@ python/rchol/util.py:32-61).  A ``Solver`` owns one engine context: A and the
 factor G are uploaded once and stay resident in HBM across solves.
 """
 import ctypes
+import hashlib
 
 import numpy as np
 import scipy.sparse as sp
@@ -49,6 +50,15 @@ class Stats(ctypes.Structure):
         d["kernel_ms"] = list(self.kernel_ms)
         d["setup_ms"] = list(self.setup_ms)
         return d
+
+
+def _pattern_key(A):
+    """Digest of a CSR matrix's shape, indptr and indices."""
+    h = hashlib.blake2b(digest_size=16)
+    h.update(np.asarray(A.shape, dtype=np.int64).tobytes())
+    h.update(np.ascontiguousarray(A.indptr, dtype=np.int64).tobytes())
+    h.update(np.ascontiguousarray(A.indices, dtype=np.int64).tobytes())
+    return h.digest()
 
 
 def _u64(a):
@@ -128,6 +138,19 @@ class Solver:
         v, pv = _f64(A.data)
         _lib.check(self._lib.rchol_amd_upload_A(self._h, n, prp, pci, pv))
         self.n = n
+        self._pattern = _pattern_key(A)
+
+    def update_A_values(self, A, check_pattern=True):
+        """New values for the resident matrix, whose sparsity pattern is
+        unchanged (the ex_reuse_partition.py sequence): one copy of the
+        values.  check_pattern compares a digest of indptr/indices with the
+        uploaded matrix's and raises ValueError on a different pattern."""
+        A = sp.csr_matrix(A)
+        A.sort_indices()
+        if check_pattern and _pattern_key(A) != getattr(self, "_pattern", None):
+            raise ValueError("sparsity pattern differs from the resident matrix")
+        v, pv = _f64(A.data)
+        _lib.check(self._lib.rchol_amd_update_A_values(self._h, v.size, pv))
 
     def upload_G(self, G, precision=FP64):
         if isinstance(G, tuple):

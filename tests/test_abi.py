@@ -52,3 +52,4 @@ def test_null_arguments_are_rejected():
     assert lib.rchol_amd_upload_A(None, 4, None, None, None) == -1
     assert lib.rchol_amd_precond(None, None, None) == -1
     assert lib.rchol_amd_set_option(None, 1, 0.0) == -1
+    assert lib.rchol_amd_update_A_values(None, 4, None) == -1

@@ -119,3 +119,32 @@ def test_device_entry_points_match_host_ones():
             assert np.array_equal(Xt.cpu().numpy(), Xh)
             with pytest.raises(RuntimeError, match="4, 8, 16 or 32 columns"):
                 s.pcg_batch_device(3, Bt.data_ptr(), Xt.data_ptr(), 1e-10, 500)
+
+
+def test_reuse_partition_sequence_updates_values_in_place():
+    # python/ex_reuse_partition.py: A, then B = A + 1e-3 I (same sparsity)
+    # factored with the first ordering; here the second solve keeps the
+    # device residency of A's pattern and only replaces its values
+    n = 20
+    A = F.laplace_3d(n)
+    N = A.shape[0]
+    b = np.random.default_rng(11).random(N)
+    G, perm, part = F.rchol(A, 2)
+    p = perm.astype(np.int64)
+    B = (A + 1e-3 * sp.identity(N)).tocsr()
+    L = F.rchol(B, 2, perm, part)[0]
+    Aperm, Bperm = A[p][:, p].tocsr(), B[p][:, p].tocsr()
+    print("case reuse_partition", flush=True)
+    with rchol_amd.Solver(Aperm, G) as s:
+        x1, r1, i1 = s.pcg(b[p], 1e-6, 200)
+        assert r1 <= 1e-6
+        s.update_A_values(Bperm)
+        s.upload_G(L)
+        x2, r2, i2 = s.pcg(b[p], 1e-6, 200)
+        with pytest.raises(ValueError, match="pattern"):
+            s.update_A_values(sp.identity(N, format="csr"))
+    with rchol_amd.Solver(Bperm, L) as fresh:
+        x3, r3, i3 = fresh.pcg(b[p], 1e-6, 200)
+    assert i2 == i3 and r2 == r3 and np.array_equal(x2, x3)
+    xo, ro, io = O.pcg(Bperm, b[p], 1e-6, 200, rchol_amd.stored_from_lower(L))
+    assert abs(i2 - io) <= 1 and rel(x2, xo) <= 1e-9
