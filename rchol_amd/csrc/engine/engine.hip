@@ -272,7 +272,8 @@ int lane_lg(int len, int lmin_lg) {
 // Tasks of the middle levels [h, t), in level order and, inside a level,
 // in slot (nested-dissection) order.  A level is wide when one-pass tasks
 // of 16 rows would outnumber the waves: its slots go to stream tasks of up
-// to 64 rows and 64 * kStream entries (one round of gathers).  A narrow
+// to 64 rows and 64 * kStream entries (one round of gathers), or, when its
+// rows read nothing (level 0), to flat tasks of 64 * kFlatPerLane.  A narrow
 // level is cut into one-pass lane tasks whose width L covers the longest of
 // its 64/L rows in one round of kGather gathers per lane, so it spreads
 // over as many waves as it can use.
@@ -294,6 +295,11 @@ void cut_tasks(const std::vector<int> &lp, const std::vector<int> &rp, int h, in
     const int s_end = lp[l + 1];
     const long long rows = s_end - lp[l];
     const bool wide = rows > 16 * nwaves;
+    if (wide && rp[s_end] == rp[lp[l]]) {  // a wide level that reads nothing: flat tasks
+      for (int s = lp[l]; s < s_end; s += 64 * kFlatPerLane)
+        push(s, std::min(s_end - s, 64 * kFlatPerLane), kTaskFlat, l);
+      continue;
+    }
     for (int s = lp[l]; s < s_end;) {
       if (wide) {
         int k = 1;  // at least one row, however long (the kernel loops rounds)

@@ -127,7 +127,9 @@ __device__ __forceinline__ void sweep_task(int cnt, int m_row, int m_e0, int m_e
 #endif
 constexpr int kStream = RCHOL_AMD_STREAM;  // entries per lane and round of a stream task
 constexpr int kTaskStream = 7;             // lane code of a stream task
-constexpr int kTaskInfoShift = 8;          // task info word: slot count | lane code << 8
+constexpr int kTaskFlat = 8;               // lane code of a flat task (see sweep_flat)
+constexpr int kFlatPerLane = 8;            // slots per lane of a flat task
+constexpr int kTaskInfoShift = 10;         // task info word: slot count | lane code << 10
 
 template <typename VT, typename XT, typename BT, int FUSE>
 __device__ __forceinline__ void sweep_stream(int cnt, int m_row, int m_e0, int m_e1, VT m_dg,
@@ -188,6 +190,39 @@ __device__ __forceinline__ void sweep_stream(int cnt, int m_row, int m_e0, int m
   }
 }
 
+// A flat task (task_lg == kTaskFlat): cnt <= 64 * kFlatPerLane slots from
+// s0 of a level whose rows read nothing (no off-diagonal entries: the rows
+// of level 0).  Every slot's row, pivot and right-hand side are loaded at
+// once, kFlatPerLane per lane, instead of one 64-row task after another.
+// Same arithmetic as a row with no entries in the other task kinds.
+template <typename VT, typename XT, typename BT, int FUSE>
+__device__ __forceinline__ void sweep_flat(int s0, int cnt, const int *__restrict__ slot_row,
+                                           const VT *__restrict__ diag,
+                                           const BT *__restrict__ rhs, XT *x,
+                                           XT *__restrict__ aux,
+                                           const double *__restrict__ dotv, double &dacc) {
+  const int lane = threadIdx.x & 63;
+  int row[kFlatPerLane];
+  VT dg[kFlatPerLane];
+  BT b[kFlatPerLane];
+#pragma unroll
+  for (int u = 0; u < kFlatPerLane; u++) {
+    const int t = u * 64 + lane;
+    row[u] = t < cnt ? ld_stream(slot_row + s0 + t) : -1;
+    dg[u] = t < cnt ? ld_stream(diag + s0 + t) : VT(1);
+  }
+#pragma unroll
+  for (int u = 0; u < kFlatPerLane; u++) b[u] = row[u] >= 0 ? ld_stream(rhs + row[u]) : (BT)0;
+#pragma unroll
+  for (int u = 0; u < kFlatPerLane; u++) {
+    if (row[u] < 0) continue;
+    const XT v = ((XT)b[u] - XT(0)) / (XT)dg[u];
+    Sent<XT>::store_agent(x + row[u], v);
+    if (FUSE == 1) aux[row[u]] = Sent<XT>::value();
+    if (FUSE == 2) dacc += dotv[row[u]] * (double)v;
+  }
+}
+
 template <typename VT, typename XT, typename BT, int FUSE>
 __global__ __launch_bounds__(256) void k_sptrsv(
     const int *__restrict__ wave_ptr, const int *__restrict__ wt_s0,
@@ -208,23 +243,24 @@ __global__ __launch_bounds__(256) void k_sptrsv(
   // metadata of task k+1 are loaded before task k runs, so a task starts
   // with its rows, entry ranges and pivots already in registers.
   int k = wave_ptr[wave];
-  int n_info = 0, n_row = 0, n_e0 = 0, n_e1 = 0;
+  int n_info = 0, n_s0 = 0, n_row = 0, n_e0 = 0, n_e1 = 0;
   VT n_dg = 1;
   auto fetch = [&](int kk) {
-    const int s = wt_s0[kk];
+    n_s0 = wt_s0[kk];
     n_info = wt_info[kk];
-    if (lane < (n_info & ((1 << kTaskInfoShift) - 1))) {
-      n_row = ld_stream(slot_row + s + lane);
-      n_e0 = ld_stream(rp + s + lane);
-      n_e1 = ld_stream(rp + s + lane + 1);
-      n_dg = ld_stream(diag + s + lane);
+    if ((n_info >> kTaskInfoShift) != kTaskFlat &&
+        lane < (n_info & ((1 << kTaskInfoShift) - 1))) {
+      n_row = ld_stream(slot_row + n_s0 + lane);
+      n_e0 = ld_stream(rp + n_s0 + lane);
+      n_e1 = ld_stream(rp + n_s0 + lane + 1);
+      n_dg = ld_stream(diag + n_s0 + lane);
     }
   };
   if (k < k1) fetch(k);
   for (; k < k1; k++) {
-    const int info = n_info;
+    const int info = n_info, m_s0 = n_s0;
     const int lg = info >> kTaskInfoShift;
-    const int cnt = info & ((1 << kTaskInfoShift) - 1);  // <= 64 slots
+    const int cnt = info & ((1 << kTaskInfoShift) - 1);  // <= 64 slots (flat: 64 kFlatPerLane)
     const int m_row = n_row, m_e0 = n_e0, m_e1 = n_e1;
     const VT m_dg = n_dg;
     if (k + 1 < k1) fetch(k + 1);
@@ -232,6 +268,9 @@ __global__ __launch_bounds__(256) void k_sptrsv(
   sweep_task<VT, XT, BT, LL, FUSE>(cnt, m_row, m_e0, m_e1, m_dg, ci, cv, rhs, x, aux, dotv, err, \
                                    coherent, dacc)
     switch (lg) {
+      case kTaskFlat:
+        sweep_flat<VT, XT, BT, FUSE>(m_s0, cnt, slot_row, diag, rhs, x, aux, dotv, dacc);
+        break;
       case kTaskStream:
         sweep_stream<VT, XT, BT, FUSE>(cnt, m_row, m_e0, m_e1, m_dg, ci, cv, rhs, x, aux, dotv, err,
                                        coherent, dacc, buf);
