@@ -273,7 +273,8 @@ int lane_lg(int len, int lmin_lg) {
 // in slot (nested-dissection) order.  A level is wide when one-pass tasks
 // of 16 rows would outnumber the waves: its slots go to stream tasks of up
 // to 64 rows and 64 * kStream entries (one round of gathers), or, when its
-// rows read nothing (level 0), to flat tasks of 64 * kFlatPerLane.  A narrow
+// rows read nothing (level 0 of the forward sweep: flat_ok), to flat tasks
+// of 64 * kFlatPerLane.  A narrow
 // level is cut into one-pass lane tasks whose width L covers the longest of
 // its 64/L rows in one round of kGather gathers per lane, so it spreads
 // over as many waves as it can use.
@@ -282,7 +283,7 @@ struct TaskCut {
 };
 
 void cut_tasks(const std::vector<int> &lp, const std::vector<int> &rp, int h, int t, int lmin_lg,
-               long long nwaves, TaskCut &tc) {
+               long long nwaves, bool flat_ok, TaskCut &tc) {
   tc.s0.clear();
   tc.info.clear();
   tc.level.clear();
@@ -295,7 +296,7 @@ void cut_tasks(const std::vector<int> &lp, const std::vector<int> &rp, int h, in
     const int s_end = lp[l + 1];
     const long long rows = s_end - lp[l];
     const bool wide = rows > 16 * nwaves;
-    if (wide && rp[s_end] == rp[lp[l]]) {  // a wide level that reads nothing: flat tasks
+    if (flat_ok && wide && rp[s_end] == rp[lp[l]]) {  // a wide level that reads nothing
       for (int s = lp[l]; s < s_end; s += 64 * kFlatPerLane)
         push(s, std::min(s_end - s, 64 * kFlatPerLane), kTaskFlat, l);
       continue;
@@ -727,7 +728,7 @@ int rchol_amd_ctx::size_trsv_grid() {
   size_t max_tasks = 0;
   for (int k = 0; k < 2; k++) {
     compute_split(sw[k]->level_ptr_h, sw[k]->nlevels, serial_width, h[k], t[k]);
-    cut_tasks(sw[k]->level_ptr_h, sw[k]->rp_h, h[k], t[k], lmin_lg, plan_waves, tc[k]);
+    cut_tasks(sw[k]->level_ptr_h, sw[k]->rp_h, h[k], t[k], lmin_lg, plan_waves, k == 0, tc[k]);
     max_tasks = std::max(max_tasks, tc[k].s0.size());
   }
   trsv_grid = (int)std::max<size_t>(1, std::min<size_t>(cap_blocks, (max_tasks + 3) / 4));

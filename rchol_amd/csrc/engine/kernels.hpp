@@ -268,8 +268,11 @@ __global__ __launch_bounds__(256) void k_sptrsv(
   sweep_task<VT, XT, BT, LL, FUSE>(cnt, m_row, m_e0, m_e1, m_dg, ci, cv, rhs, x, aux, dotv, err, \
                                    coherent, dacc)
     switch (lg) {
-      case kTaskFlat:
-        sweep_flat<VT, XT, BT, FUSE>(m_s0, cnt, slot_row, diag, rhs, x, aux, dotv, dacc);
+      case kTaskFlat:  // forward sweeps only (FUSE 0 / 1): keeps the r.z kernel's registers
+        if constexpr (FUSE != 2)
+          sweep_flat<VT, XT, BT, FUSE>(m_s0, cnt, slot_row, diag, rhs, x, aux, dotv, dacc);
+        else
+          atomicOr(err, 2);  // not in a backward schedule
         break;
       case kTaskStream:
         sweep_stream<VT, XT, BT, FUSE>(cnt, m_row, m_e0, m_e1, m_dg, ci, cv, rhs, x, aux, dotv, err,
