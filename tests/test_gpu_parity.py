@@ -1,7 +1,7 @@
 """GPU parity of the HIP path against the oracle (CPU restatement of
 c++/util/pcg.cpp), through the C ABI.
 
-Bars (BASELINE north star): FP64 preconditioner application within 1e-12
+Bars: FP64 preconditioner application within 1e-12
 relative (2-norm) of the oracle; PCG iteration count to 1e-10 within +-1 of
 the oracle; SpMV within 1e-13 relative.  The FP32 factor path is held to
 1e-5 relative on the application and to an FP64-verified 1e-10 solve.

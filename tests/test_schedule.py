@@ -3,7 +3,7 @@
 The persistent sweep kernel is deadlock-free only if every dependency of a
 row lies in a strictly lower level of its sweep and the schedule covers every
 row exactly once; the split head/tail phases need levels of at most the
-threshold width.  Checked on the reference fixtures and on this build's
+threshold width.  Checked on the golden fixtures and on the package's
 factors through rchol_amd_analyze_G (the code upload_G runs)."""
 import glob
 import os
