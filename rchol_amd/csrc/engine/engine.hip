@@ -58,9 +58,11 @@ constexpr int kTaskRows = 64;     // slots per triangular-sweep task
 #define RCHOL_AMD_SPMV_PER_CU 2
 #endif
 constexpr int kSpmvBlocksPerCu = RCHOL_AMD_SPMV_PER_CU;
-// The K-column SpMV gathers K values per entry and needs more rows in flight.
+// The K-column SpMV gathers K values per entry and needs more rows in flight
+// (K = 32 at 256^3, ms per SpMV: 4 per CU 12.7, 5 11.5, 6 10.9, 7 11.1,
+// 8 12.1).
 #ifndef RCHOL_AMD_SPMV_MR_PER_CU
-#define RCHOL_AMD_SPMV_MR_PER_CU 8
+#define RCHOL_AMD_SPMV_MR_PER_CU 6
 #endif
 constexpr int kSpmvMrBlocksPerCu = RCHOL_AMD_SPMV_MR_PER_CU;
 #ifndef RCHOL_AMD_VEC_PER_CU
