@@ -1056,7 +1056,7 @@ void rchol_amd_ctx::launch_iteration_mr(int parity, int slot) {
       1, std::min<size_t>((n + rows_per_block - 1) / rows_per_block, (size_t)ncu * kSpmvMrBlocksPerCu));
   if (sg > 8) sg = (sg + 7) / 8 * 8;  // whole XCD rounds for the row walk
   const int vg = (int)std::max<size_t>(
-      1, std::min<size_t>((n * K + kBlock - 1) / kBlock, (size_t)ncu * 8));
+      1, std::min<size_t>((n * K + kBlock - 1) / kBlock, (size_t)ncu * kVecBlocksPerCu));
   if (gprec == RCHOL_AMD_FP32) {
     hipLaunchKernelGGL((k_spmv_p_mr<float, K>), dim3(sg), dim3(kBlock), 0, stream, (int)n,
                        arp.p, aci.p, av.p, (const float *)batch.z32.p, (const double *)p_old,
@@ -1098,7 +1098,7 @@ int rchol_amd_ctx::pcg_batch(const double *d_b, int ncols, double tol, int maxit
   HIP_TRY(hipMemcpyAsync(&batch.bs->ncols, &batch.hbs->ncols, sizeof(int), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(&batch.bs->zero, &batch.hbs->zero, sizeof(int), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipEventRecord(ev0, stream));
-  const int vg = (int)std::max<size_t>(1, std::min<size_t>((m + kBlock - 1) / kBlock, (size_t)ncu * 8));
+  const int vg = (int)std::max<size_t>(1, std::min<size_t>((m + kBlock - 1) / kBlock, (size_t)ncu * kVecBlocksPerCu));
   if (gprec == RCHOL_AMD_FP32)
     hipLaunchKernelGGL((k_init_mr<float, K>), dim3(vg), dim3(kBlock), 0, stream, (int)n, d_b,
                        batch.r.p, batch.x.p, keep_x, batch.y32.p, batch.partials.p, batch.bs);
