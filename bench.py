@@ -481,7 +481,11 @@ def run_c5(args):
             x = AG.solve(w, bt, prob)
             ((x - xt) ** 2).sum().backward()
 
-        step()  # factor, upload and graph capture outside the timed region
+        # untimed: the first step factors, uploads and captures the graph;
+        # the next ones run the cached-factor path once (its first use of
+        # torch kernels loads their code)
+        for _ in range(max(args.warmup, 1) + 1):
+            step()
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(max(args.steps, 1)):
@@ -493,7 +497,7 @@ def run_c5(args):
             f"{times[name]:.3f}s")
     g, gr = grads["mixed"], grads["fp64"]
     res = _base("differentiable solve: forward + adjoint time, 100^3 graph Laplacian",
-                times["mixed"], "s", max(args.steps, 1), 1, False,
+                times["mixed"], "s", max(args.steps, 1), max(args.warmup, 1) + 1, False,
                 {"workload": f"{side}^3 6-neighbour graph Laplacian, w = softplus(theta), "
                              "+1e-2 I, loss ||x - x*||^2, mixed forward/adjoint PCG",
                  "n": n, "edges": int(g.size)}, "f64 + f32 factor")
