@@ -280,6 +280,11 @@ int lane_lg(int len, int lmin_lg) {
 // level is cut into one-pass lane tasks whose width L covers the longest of
 // its 64/L rows in one round of kGather gathers per lane, so it spreads
 // over as many waves as it can use.
+#ifndef RCHOL_AMD_WIDE_ROWS
+#define RCHOL_AMD_WIDE_ROWS 16
+#endif
+constexpr long long kWideRowsPerWave = RCHOL_AMD_WIDE_ROWS;  // stream-task threshold
+
 struct TaskCut {
   std::vector<int> s0, info, level;  // info = cnt | lg << kTaskInfoShift
 };
@@ -297,7 +302,7 @@ void cut_tasks(const std::vector<int> &lp, const std::vector<int> &rp, int h, in
   for (int l = h; l < t; l++) {
     const int s_end = lp[l + 1];
     const long long rows = s_end - lp[l];
-    const bool wide = rows > 16 * nwaves;
+    const bool wide = rows > kWideRowsPerWave * nwaves;
     if (flat_ok && wide && rp[s_end] == rp[lp[l]]) {  // a wide level that reads nothing
       for (int s = lp[l]; s < s_end; s += 64 * kFlatPerLane)
         push(s, std::min(s_end - s, 64 * kFlatPerLane), kTaskFlat, l);
