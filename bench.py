@@ -206,6 +206,15 @@ def run_c3(args):
                           f"{it64} iterations of the FP64 solve"),
                "s_per_iter": per_it}
         log(f"CPU: {per_it:.2f} s/iteration -> {per_it * it64:.1f} s")
+        # the same sample on one core (the sweeps are sequential either way)
+        O.set_threads(1)
+        t = time.perf_counter()
+        O.pcg(Ap, bp, 1e-30, args.cpu_iters, Gs)
+        per_it1 = (time.perf_counter() - t) / args.cpu_iters
+        O.set_threads(threads)
+        cpu["one_core"] = {"value": per_it1 * it64, "unit": "s", "cores": 1,
+                           "s_per_iter": per_it1}
+        log(f"CPU, one core: {per_it1:.2f} s/iteration -> {per_it1 * it64:.1f} s")
 
     res = {
         "metric": "PCG time-to-1e-10 & HBM GB/s, 256^3 var-coef Poisson; mixed vs FP64 GPU",
