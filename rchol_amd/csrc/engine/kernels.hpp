@@ -131,6 +131,22 @@ constexpr int kTaskFlat = 8;               // lane code of a flat task (see swee
 constexpr int kFlatPerLane = 8;            // slots per lane of a flat task
 constexpr int kTaskInfoShift = 10;         // task info word: slot count | lane code << 10
 
+// acc + buf[lo] + buf[lo + 1] + ... + buf[hi - 1], added in that order, with
+// four LDS reads in flight at a time (a plain loop waits for each read).
+template <typename T>
+__device__ __forceinline__ T lds_row_sum(const T *buf, int lo, int hi, T acc) {
+  int e = lo;
+  for (; e + 4 <= hi; e += 4) {
+    const T a0 = buf[e], a1 = buf[e + 1], a2 = buf[e + 2], a3 = buf[e + 3];
+    acc += a0;
+    acc += a1;
+    acc += a2;
+    acc += a3;
+  }
+  for (; e < hi; e++) acc += buf[e];
+  return acc;
+}
+
 template <typename VT, typename XT, typename BT, int FUSE>
 __device__ __forceinline__ void sweep_stream(int cnt, int m_row, int m_e0, int m_e1, VT m_dg,
                                              const int *__restrict__ ci,
@@ -176,7 +192,7 @@ __device__ __forceinline__ void sweep_stream(int cnt, int m_row, int m_e0, int m
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (live) {
       const int lo = max(m_e0, c) - c, hi = min(m_e1, c + 64 * kStream) - c;
-      for (int e = lo; e < hi; e++) acc += buf[e];
+      acc = lds_row_sum(buf, lo, hi, acc);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -675,7 +691,7 @@ __device__ __forceinline__ double stream_rows(int r0, int n, const int *__restri
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (live) {
       const int lo = max(e0, c) - c, hi = min(e1, c + 64 * kSpmvStream) - c;
-      for (int e = lo; e < hi; e++) acc += buf[e];
+      acc = lds_row_sum(buf, lo, hi, acc);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
