@@ -13,6 +13,7 @@
   the same kernels run, so the results are bit-identical; a device batch
   that is not 4, 8, 16 or 32 columns wide is refused.
 """
+import ctypes
 import os
 import re
 import subprocess
@@ -143,6 +144,11 @@ def test_reuse_partition_sequence_updates_values_in_place():
         x2, r2, i2 = s.pcg(b[p], 1e-6, 200)
         with pytest.raises(ValueError, match="pattern"):
             s.update_A_values(sp.identity(N, format="csr"))
+        # the C ABI itself refuses a value count that is not the resident nnz
+        short = np.ones(Bperm.nnz - 1)
+        assert s._lib.rchol_amd_update_A_values(
+            s._h, short.size, short.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == -1
+        assert b"differs from the resident" in s._lib.rchol_amd_last_error()
     with rchol_amd.Solver(Bperm, L) as fresh:
         x3, r3, i3 = fresh.pcg(b[p], 1e-6, 200)
     assert i2 == i3 and r2 == r3 and np.array_equal(x2, x3)
