@@ -11,6 +11,8 @@ dL/dw_e = -(lam_u - lam_v)(x_u - x_v).
 PyTorch appears only here, for device memory and autograd plumbing; the
 solve and the edge gradient are the engine's HIP kernels (C ABI).
 """
+import hashlib
+
 import numpy as np
 import torch
 
@@ -47,23 +49,25 @@ class GridLaplacian:
         return F.grid_laplacian(*self.dims, np.asarray(w, dtype=np.float64), self.shift)
 
     def solver(self, w):
-        """Solver for A(w) (re-factored when w changes)."""
+        """Solver for A(w) (re-factored when w changes).
+
+        The cache holds only the latest solver.  A replaced solver is not
+        closed here: a forward pass whose backward is still pending keeps
+        its own solver alive through its autograd context, and the context
+        is freed (closing the solver) once that backward has run."""
         w = np.ascontiguousarray(w, dtype=np.float64)
-        key = hash(w.tobytes())
+        key = hashlib.blake2b(w.tobytes(), digest_size=32).digest()
         if key != self._key:
             A = self.matrix(w)
             perm, blocks = F.nd_grid(*self.dims, self.depth)
             Gs, po, _ = F.factor(A, perm, blocks, self.order, self.seed)
             P = po.astype(np.int64)
             Ap = A[P][:, P].tocsr()
-            if self._solver is not None:
-                self._solver.close()
             s = Solver(Ap, Gs, self.precision, device=self.device)
             s.set_perm(po)
             self._solver = s
             self._key = key
         return self._solver
-
 
     def solver_for(self, w):
         """Solver for the weights of a device tensor: the cached factor is
@@ -92,6 +96,9 @@ class RcholSolve(torch.autograd.Function):
         relres, itr = s.pcg_device(b.data_ptr(), x.data_ptr(), problem.tol, problem.maxit)
         problem.last["forward"] = (relres, itr)
         ctx.problem = problem
+        # the adjoint must use the factor of A(w) for this w, not whatever
+        # the problem's cache holds when backward runs
+        ctx.solver = s
         ctx.save_for_backward(x)
         return x
 
@@ -99,7 +106,7 @@ class RcholSolve(torch.autograd.Function):
     def backward(ctx, gx):
         (x,) = ctx.saved_tensors
         problem = ctx.problem
-        s = problem._solver
+        s = ctx.solver
         gx = gx.contiguous()
         lam = torch.empty_like(gx)
         torch.cuda.current_stream(gx.device).synchronize()

@@ -1,7 +1,9 @@
 // The reference's C++ driver (c++/ex_laplace.cpp, ex_laplace_parallel.cpp)
 // on this engine: 3D Poisson, seeded RHS, rchol factor, PCG through the
 // drop-in pcg(...) constructor.  Usage: ex_laplace -n SIDE [-t LEAVES]
-// [-tol TOL] [-maxit M] [-seed S]
+// [-tol TOL] [-maxit M] [-seed S] [-bfile B] [-xfile X]
+// -bfile reads b (SIDE^3 raw FP64, original numbering) instead of drawing it;
+// -xfile writes the solution x (raw FP64, original numbering).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -25,7 +27,10 @@ int main(int argc, char **argv) {
   int n = 3, threads = 1, maxit = 200;
   unsigned seed = 1;
   double tol = 1e-6;
+  const char *bfile = nullptr, *xfile = nullptr;
   for (int i = 1; i + 1 < argc; i++) {
+    if (!strcmp(argv[i], "-bfile")) bfile = argv[i + 1];
+    if (!strcmp(argv[i], "-xfile")) xfile = argv[i + 1];
     if (!strcmp(argv[i], "-n")) n = atoi(argv[i + 1]);
     if (!strcmp(argv[i], "-t")) threads = atoi(argv[i + 1]);
     if (!strcmp(argv[i], "-tol")) tol = atof(argv[i + 1]);
@@ -39,6 +44,14 @@ int main(int argc, char **argv) {
   std::mt19937_64 gen(seed);
   std::uniform_real_distribution<double> dis(-1.0, 1.0);
   for (auto &v : b) v = dis(gen);
+  if (bfile) {
+    FILE *fb = fopen(bfile, "rb");
+    if (!fb || fread(b.data(), sizeof(double), N, fb) != N) {
+      fprintf(stderr, "cannot read %zu doubles from %s\n", N, bfile);
+      return 1;
+    }
+    fclose(fb);
+  }
   // nested dissection with `threads` leaves (identity ordering for 1)
   int depth = 1;
   while ((1 << (depth - 1)) < threads) depth++;
@@ -96,6 +109,17 @@ int main(int argc, char **argv) {
   }
   printf("# CG iterations: %d\n", itr);
   printf("Relative residual: %.3g\n", relres);
+  if (xfile) {
+    // x of A(P,P) back to the original numbering: x_orig[P[i]] = x[i]
+    std::vector<double> xo(N);
+    for (size_t i = 0; i < N; i++) xo[f.perm[i]] = x[i];
+    FILE *fx = fopen(xfile, "wb");
+    if (!fx || fwrite(xo.data(), sizeof(double), N, fx) != N) {
+      fprintf(stderr, "cannot write %s\n", xfile);
+      return 1;
+    }
+    fclose(fx);
+  }
   rchol_factor_free(f.colPtr);
   rchol_factor_free(f.rowIdx);
   rchol_factor_free(f.val);

@@ -7,8 +7,10 @@
   same inputs and against the unpermuted system.
 - C++: the pcg(A, b, tol, maxit, G, x, relres, itr) constructor of
   c++/util/pcg.hpp:15-16 (include/rchol_amd_pcg.hpp), through the example
-  driver that mirrors c++/ex_laplace.cpp, against the Python path on the
-  same factor.
+  driver that mirrors c++/ex_laplace.cpp, against the oracle on the same
+  factor and b (x and the iteration count); and the reference's own
+  c++/ex_laplace.cpp, compiled unchanged against a forwarding pcg.hpp,
+  against the oracle on its own seeded factor and right-hand side.
 - The device-pointer entry points (torch tensors) against their host twins:
   the same kernels run, so the results are bit-identical; a device batch
   that is not 4, 8, 16 or 32 columns wide is refused.
@@ -62,30 +64,78 @@ def test_python_dropin_follows_ex_laplace():
     assert true <= 2e-10
 
 
-def test_cpp_pcg_shim_example():
+def test_cpp_pcg_shim_example(tmp_path):
+    """The pcg(...) constructor of include/rchol_amd_pcg.hpp through the
+    example driver, against the oracle on the same factor and the same b."""
     exe = os.path.join(ROOT, "rchol_amd", "csrc", "bin", "ex_laplace")
     assert os.path.exists(exe), "the C++ example is built by make -C rchol_amd/csrc all"
-    side, leaves, seed = 16, 4, 1
-    print("case cpp_shim", flush=True)
-    out = subprocess.run([exe, "-n", str(side), "-t", str(leaves), "-tol", "1e-10",
-                          "-maxit", "500", "-seed", str(seed)],
+    side, leaves, seed, tol = 16, 4, 1, 1e-10
+    print("case cpp_shim vs oracle", flush=True)
+    n = side ** 3
+    b = np.random.default_rng(5).uniform(-1.0, 1.0, n)
+    bfile, xfile = tmp_path / "b.bin", tmp_path / "x.bin"
+    b.tofile(bfile)
+    out = subprocess.run([exe, "-n", str(side), "-t", str(leaves), "-tol", str(tol),
+                          "-maxit", "500", "-seed", str(seed), "-bfile", str(bfile),
+                          "-xfile", str(xfile)],
                          capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     fill = float(re.search(r"Fill-in ratio: (\S+)", out.stdout).group(1))
     itr = int(re.search(r"# CG iterations: (\d+)", out.stdout).group(1))
     relres = float(re.search(r"Relative residual: (\S+)", out.stdout).group(1))
-    # the same factor from the Python side: static order, ND with 4 leaves
+    x = np.fromfile(xfile, dtype=np.float64)
+    # the same factor on the host: static order, ND with 4 leaves, same seed
     A = F.laplace_3d(side)
     perm, blocks = F.nd_grid(side, side, side, 3)
     Gs, po, _ = F.factor(A, perm, blocks, "static", seed)
+    P = po.astype(np.int64)
     Ap = F.permute(A, po)
     assert fill == pytest.approx(2.0 * Gs[0][-1] / Ap.nnz, rel=2e-3)
-    b = np.random.default_rng(5).uniform(-1.0, 1.0, A.shape[0])
-    with rchol_amd.Solver(Ap, Gs) as s:
-        _, rr_py, it_py = s.pcg(b, 1e-10, 500)
-    # a different right-hand side (the driver draws b with std::mt19937_64)
-    assert abs(itr - it_py) <= 3
-    assert 0.0 < relres <= 2e-10 and rr_py <= 2e-10
+    xo_p, ro, io = O.pcg(Ap, b[P], tol, 500, Gs)
+    xo = np.empty(n)
+    xo[P] = xo_p
+    print(f"cpp shim: itr {itr} oracle {io}, x {rel(x, xo):.1e}", flush=True)
+    assert abs(itr - io) <= 1
+    assert rel(x, xo) <= 1e-8
+    assert 0.0 < relres <= 2 * tol
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) <= 2 * tol
+
+
+def test_reference_driver_unchanged_on_the_dropin():
+    """The reference's own c++/ex_laplace.cpp, compiled unchanged with the
+    forwarding pcg.hpp (oracle/Makefile target dropin), solves on this
+    engine: its factor (rchol, seeded through the driver's random_device)
+    and right-hand side (util.hpp:50-54 rand) are rebuilt on the host and
+    the oracle runs the same system."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from std_random import reference_rand
+    exe = os.path.join(ROOT, "oracle", "_ref", "ex_laplace_dropin")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/ex_laplace_dropin not built (needs the reference sources)")
+    if not O.ref_available():
+        pytest.skip("oracle/_ref/librchol_ref.so not built")
+    for side, seed in ((12, 3), (20, 11)):
+        print(f"case reference driver n={side} seed={seed}", flush=True)
+        env = dict(os.environ, RCHOL_DROPIN_SEED=str(seed))
+        out = subprocess.run([exe, "-n", str(side)], capture_output=True, text=True,
+                             timeout=120, env=env)
+        assert out.returncode == 0, out.stderr
+        fill = float(re.search(r"Fill-in ratio: (\S+)", out.stdout).group(1))
+        itr = int(re.search(r"# CG iterations: (\d+)", out.stdout).group(1))
+        relres = float(re.search(r"Relative residual: (\S+)", out.stdout).group(1))
+        A = O.ref_laplace_3d(side)
+        N = A.shape[0]
+        Gs = O.ref_rchol_seq(A, seed)
+        b = reference_rand(N, seed)
+        assert fill == pytest.approx(2.0 * Gs[0][-1] / A.nnz, rel=2e-3)
+        xo, ro, io = O.pcg(A, b, 1e-6, 200, Gs)   # ex_laplace.cpp:37-38
+        print(f"reference driver: itr {itr} oracle {io}, relres {relres:.3g} oracle {ro:.3g}",
+              flush=True)
+        assert abs(itr - io) <= 1
+        assert relres <= 1e-6
+        if itr == io:
+            assert relres == pytest.approx(ro, rel=1e-2)
 
 
 def test_device_entry_points_match_host_ones():
