@@ -149,6 +149,20 @@ int rchol_amd_pcg_device(rchol_amd_ctx *ctx, const double *d_b, double tol, int 
 /* Preconditioner apply on device pointers. */
 int rchol_amd_precond_device(rchol_amd_ctx *ctx, const double *d_r, double *d_z);
 
+/* rchol_amd_precond_device without any device allocation: no counterpart in
+ * the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc): the
+ * sweep vectors, the permutation staging and the control block live in a
+ * static 64 MiB device array that each GPU holds until the process ends.
+ * Limit: y (and z for an FP32 factor) take 8 bytes per row, plus 16 per row
+ * with a permutation set: about 8.4 M rows, or 2.8 M with a permutation;
+ * larger factors get RCHOL_AMD_ERANGE.  Every call on one GPU shares that
+ * GPU's array (it keeps the last call's contents; each call resets what it
+ * reads), so calls that overlap on one GPU -- from several threads or
+ * contexts -- race and must be serialised by the caller.  With an FP64
+ * factor and no permutation, d_r and d_z must not overlap. */
+int rchol_amd_precond_device_noalloc(rchol_amd_ctx *ctx, const double *d_r, double *d_z);
+
 /* Batched PCG on k right-hand sides (1 <= k <= 32), stored interleaved:
  * B[i*k + c] is row i of column c (row-major n x k), X likewise.  Every
  * column runs the recurrence of rchol_amd_pcg with its own scalars and stops

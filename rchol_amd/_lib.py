@@ -39,6 +39,8 @@ ENGINE_SYMBOLS = {
     "rchol_amd_precond": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p]),
     "rchol_amd_precond_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                                 ctypes.c_void_p]),
+    "rchol_amd_precond_device_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_void_p]),
     "rchol_amd_spmv": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p]),
     "rchol_amd_pcg": (ctypes.c_int, [ctypes.c_void_p, f64p, ctypes.c_double, ctypes.c_int, f64p,
                                      ctypes.c_int, f64p, i32p]),

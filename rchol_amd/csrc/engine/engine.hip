@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -16,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -26,6 +28,15 @@
 #include "kernels_seg.hpp"
 
 using namespace rchol_amd;
+
+// Scratch of the allocation-free preconditioner entry point
+// (rchol_amd_precond_device_noalloc): a control block followed by the sweep
+// vectors (layout in scratch_layout()).  Each GPU holds one copy for the life
+// of the process.  Host code never passes the array's name to a kernel or a
+// copy: its address on a device is looked up once, at the first
+// rchol_amd_create on that device, and that address is used from then on.
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
 
@@ -71,6 +82,9 @@ constexpr int kSpmvMrBlocksPerCu = RCHOL_AMD_SPMV_MR_PER_CU;
 constexpr int kVecBlocksPerCu = RCHOL_AMD_VEC_PER_CU;
 constexpr int kBlock = 256;       // threads per block for every kernel
 constexpr size_t kIdxMax = 0x7FFFFFFFu;
+constexpr size_t kScratchCtl = 256;   // scratch: DevScalars of the standalone sweeps
+constexpr int kMaxDevices = 64;
+static_assert(sizeof(DevScalars) <= kScratchCtl, "control block outgrew its slot");
 
 template <typename T>
 struct DevBuf {
@@ -593,40 +607,43 @@ namespace {
 // FUSE 1: sentinel into aux; 2: r.z -> beta; 0: plain.
 template <typename VT, typename XT, typename BT, int FUSE>
 void launch_one_sweep(rchol_amd_ctx *c, const Sweep &sw, const VT *cv, const VT *dg,
-                      const BT *rhs, XT *x, XT *aux, const double *dotv, const int *stop) {
+                      const BT *rhs, XT *x, XT *aux, const double *dotv, const int *stop,
+                      DevScalars *sc) {
   const bool head = sw.head_end > 0, tail = sw.tail_begin < sw.nlevels;
   if (head)
     hipLaunchKernelGGL((k_sptrsv_serial<VT, XT, BT, FUSE>), dim3(1), dim3(1024), 0, c->stream, 0,
                        sw.head_end, sw.level_ptr.p, sw.level_lg.p, sw.slot_row.p, sw.rp.p,
-                       sw.ci.p, cv, dg, rhs, x, aux, dotv, c->sc, stop, 0);
+                       sw.ci.p, cv, dg, rhs, x, aux, dotv, sc, stop, 0);
   if (sw.mid_ntasks > 0 || (FUSE == 2 && !tail))
     hipLaunchKernelGGL((k_sptrsv<VT, XT, BT, FUSE>), dim3(c->trsv_grid), dim3(kBlock), 0,
                        c->stream, sw.wave_ptr.p, sw.wt_s0.p, sw.wt_info.p,
                        sw.slot_row.p, sw.rp.p, sw.ci.p, cv, dg, rhs, x, aux, dotv, c->partials.p,
-                       c->sc, stop, tail ? 1 : 0, c->coherent_gather);
+                       sc, stop, tail ? 1 : 0, c->coherent_gather);
   if (tail)
     hipLaunchKernelGGL((k_sptrsv_serial<VT, XT, BT, FUSE>), dim3(1), dim3(1024), 0, c->stream,
                        sw.tail_begin, sw.nlevels, sw.level_ptr.p, sw.level_lg.p, sw.slot_row.p,
-                       sw.rp.p, sw.ci.p, cv, dg, rhs, x, aux, dotv, c->sc, stop, 1);
+                       sw.rp.p, sw.ci.p, cv, dg, rhs, x, aux, dotv, sc, stop, 1);
 }
 
 template <typename VT, typename XT>
 int dispatch_sweeps(rchol_amd_ctx *c, const double *rhs, XT *y, XT *z, const int *stop,
-                    bool with_dot, const double *dotv, hipEvent_t *ev = nullptr) {
+                    bool with_dot, const double *dotv, hipEvent_t *ev = nullptr,
+                    DevScalars *sc = nullptr) {
+  if (!sc) sc = c->sc;
   const VT *fcv, *fdg, *bcv, *bdg;
   if constexpr (sizeof(VT) == 8) {
     fcv = c->fwd.cv64.p; fdg = c->fwd.diag64.p; bcv = c->bwd.cv64.p; bdg = c->bwd.diag64.p;
   } else {
     fcv = c->fwd.cv32.p; fdg = c->fwd.diag32.p; bcv = c->bwd.cv32.p; bdg = c->bwd.diag32.p;
   }
-  launch_one_sweep<VT, XT, double, 1>(c, c->fwd, fcv, fdg, rhs, y, z, nullptr, stop);
+  launch_one_sweep<VT, XT, double, 1>(c, c->fwd, fcv, fdg, rhs, y, z, nullptr, stop, sc);
   if (ev) (void)hipEventRecord(ev[1], c->stream);
   if (with_dot)
     launch_one_sweep<VT, XT, XT, 2>(c, c->bwd, bcv, bdg, (const XT *)y, z, (XT *)nullptr, dotv,
-                                    stop);
+                                    stop, sc);
   else
     launch_one_sweep<VT, XT, XT, 0>(c, c->bwd, bcv, bdg, (const XT *)y, z, (XT *)nullptr, nullptr,
-                                    stop);
+                                    stop, sc);
   if (ev) (void)hipEventRecord(ev[2], c->stream);
   return RCHOL_AMD_OK;
 }
@@ -1417,6 +1434,63 @@ int rchol_amd_ctx::pcg_segments(const double *d_b, double tol, int maxit, double
   return RCHOL_AMD_OK;
 }
 
+namespace {
+
+// scratch's address on each device (null until looked up there)
+std::atomic<unsigned char *> g_scratch_dev[kMaxDevices];
+std::mutex g_scratch_mu;
+
+// Look up scratch's address on `device`, which must be current; once per
+// device.
+int scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(RCHOL_AMD_EINVAL, "device %d beyond the %d the engine tracks", device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[device].load()) return RCHOL_AMD_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(RCHOL_AMD_EHIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", device,
+                hipGetErrorString(e));
+  g_scratch_dev[device].store((unsigned char *)p);
+  return RCHOL_AMD_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Byte offsets inside scratch for an n-row factor: [control block | y | z
+// (FP32 factor only; FP64 sweeps write z into the caller's output) | r(P),
+// z(P) staging (when a permutation is set)].  Returns the bytes needed.
+struct ScratchLayout {
+  size_t y = 0, z = 0, pin = 0, pout = 0, bytes = 0;
+};
+ScratchLayout scratch_layout(size_t n, bool fp32, bool perm) {
+  ScratchLayout L;
+  size_t o = kScratchCtl;
+  L.y = o;
+  o += align256(n * (fp32 ? sizeof(float) : sizeof(double)));
+  if (fp32) {
+    L.z = o;
+    o += align256(n * sizeof(float));
+  }
+  if (perm) {
+    L.pin = o;
+    o += align256(n * sizeof(double));
+    L.pout = o;
+    o += align256(n * sizeof(double));
+  }
+  L.bytes = o;
+  return L;
+}
+
+// Zero the control block (a kernel, not a memset, so a captured graph can
+// hold it): no error, stop flag 0, counters 0.
+__global__ void k_clear_ctl(unsigned int *ctl, int words) {
+  for (int i = threadIdx.x; i < words; i += blockDim.x) ctl[i] = 0u;
+}
+
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -1451,6 +1525,8 @@ int rchol_amd_create(int device, rchol_amd_ctx **out) {
   memset(c->hsc, 0, sizeof(DevScalars));
   HIP_TRY(hipEventCreate(&c->ev0));
   HIP_TRY(hipEventCreate(&c->ev1));
+  const int rc = scratch_init(device);
+  if (rc) return rc;
   *out = c.release();
   return RCHOL_AMD_OK;
 }
@@ -1837,6 +1913,64 @@ int rchol_amd_precond_device(rchol_amd_ctx *c, const double *d_r, double *d_z) {
   if (!rc) rc = permute_rows(c, false, c->pout.p, d_z, 1);
   if (!rc) HIP_TRY(hipStreamSynchronize(c->stream));
   return rc;
+}
+
+int rchol_amd_precond_device_noalloc(rchol_amd_ctx *c, const double *d_r, double *d_z) {
+  if (!c || !d_r || !d_z) return fail(RCHOL_AMD_EINVAL, "null argument");
+  if (!c->hasG) return fail(RCHOL_AMD_ESTATE, "G must be uploaded before precond");
+  const size_t n = c->nG;
+  if (c->hasA && c->n != n)
+    return fail(RCHOL_AMD_EINVAL, "G is %zu x %zu but A is %zu x %zu", n, n, c->n, c->n);
+  const bool fp32 = c->gprec == RCHOL_AMD_FP32;
+  const ScratchLayout L = scratch_layout(n, fp32, c->hasPerm);
+  if (L.bytes > kScratchBytes) {
+    const size_t per_row = 8 + (c->hasPerm ? 16 : 0);  // y (+ z in FP32) + staging
+    return fail(RCHOL_AMD_ERANGE,
+                "n=%zu needs %zu bytes of scratch; the allocation-free entry point holds %zu "
+                "(at most about %zu rows for this factor)",
+                n, L.bytes, kScratchBytes, (kScratchBytes - kScratchCtl - 1024) / per_row);
+  }
+  if (!c->hasPerm && !fp32 && d_z < d_r + n && d_r < d_z + n)
+    return fail(RCHOL_AMD_EINVAL, "d_r and d_z overlap (the FP64 sweeps write z in place)");
+  HIP_TRY(hipSetDevice(c->device));
+  unsigned char *base = g_scratch_dev[c->device].load();
+  if (!base) return fail(RCHOL_AMD_ESTATE, "scratch not initialised on device %d", c->device);
+  DevScalars *ctl = (DevScalars *)base;
+  hipLaunchKernelGGL(k_clear_ctl, dim3(1), dim3(64), 0, c->stream, (unsigned int *)ctl,
+                     (int)(sizeof(DevScalars) / sizeof(unsigned int)));
+  const double *rhs = d_r;
+  double *zout = d_z;
+  if (c->hasPerm) {
+    double *pin = (double *)(base + L.pin), *pout = (double *)(base + L.pout);
+    int rc = permute_rows(c, true, d_r, pin, 1);
+    if (rc) return rc;
+    rhs = pin;
+    zout = pout;
+  }
+  const int g = (int)std::max<size_t>(
+      1, std::min<size_t>((n + kBlock - 1) / kBlock, (size_t)c->ncu * kVecBlocksPerCu));
+  if (fp32) {
+    float *y = (float *)(base + L.y), *z = (float *)(base + L.z);
+    hipLaunchKernelGGL((k_fill_sentinel<float>), dim3(g), dim3(kBlock), 0, c->stream, (int)n, y);
+    dispatch_sweeps<float, float>(c, rhs, y, z, &ctl->zero, false, nullptr, nullptr, ctl);
+    hipLaunchKernelGGL(k_widen, dim3(g), dim3(kBlock), 0, c->stream, (int)n, (const float *)z,
+                       zout);
+  } else {
+    double *y = (double *)(base + L.y);
+    hipLaunchKernelGGL((k_fill_sentinel<double>), dim3(g), dim3(kBlock), 0, c->stream, (int)n, y);
+    dispatch_sweeps<double, double>(c, rhs, y, zout, &ctl->zero, false, nullptr, nullptr, ctl);
+  }
+  if (c->hasPerm) {
+    int rc = permute_rows(c, false, zout, d_z, 1);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(&c->hsc->err, &ctl->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->hsc->err)
+    return fail(RCHOL_AMD_ESTALL, "triangular sweep wait exceeded its bound (err=%d)",
+                c->hsc->err);
+  return RCHOL_AMD_OK;
 }
 
 int rchol_amd_set_perm(rchol_amd_ctx *c, size_t n, const size_t *perm) {

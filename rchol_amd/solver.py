@@ -300,6 +300,12 @@ class Solver:
         _lib.check(self._lib.rchol_amd_precond_device(self._h, ctypes.c_void_p(r_ptr),
                                                       ctypes.c_void_p(z_ptr)))
 
+    def precond_device_noalloc(self, r_ptr, z_ptr):
+        """precond_device with no device allocation: the engine's static
+        64 MiB scratch per GPU (calls on one GPU must not overlap)."""
+        _lib.check(self._lib.rchol_amd_precond_device_noalloc(self._h, ctypes.c_void_p(r_ptr),
+                                                              ctypes.c_void_p(z_ptr)))
+
     def stats(self):
         s = Stats()
         _lib.check(self._lib.rchol_amd_get_stats(self._h, ctypes.byref(s)))
