@@ -8,8 +8,10 @@ b already resident in HBM.  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--side 256] [--leaves 128]
 
-The CPU baseline is the oracle (C restatement of c++/util/pcg.cpp) timed on
-a bounded sample of the same system on this host's cores.
+The CPU baseline is the oracle (C restatement of c++/util/pcg.cpp): one full
+FP64 solve of the same system on 16 host threads (and on one core), which
+also gives the iteration and preconditioner parity at full size.  The same
+line reports a 32-right-hand-side batch on the headline factor.
 """
 import argparse
 import json
@@ -54,8 +56,11 @@ def parse():
     ap.add_argument("--order", default="mis", choices=["mis", "static"])
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--maxit", type=int, default=2000)
-    ap.add_argument("--cpu-iters", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the oracle (CPU baseline, parity)")
+    ap.add_argument("--no-cpu-one-core", action="store_true")
+    ap.add_argument("--batch-cols", type=int, default=32,
+                    help="right-hand sides of the batch run on the headline factor (0 = skip)")
+    ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--no-ref", action="store_true", help="skip the 1e-14 FP64 reference solve")
     ap.add_argument("--graph-iters", type=int, default=16)
     ap.add_argument("--trsv-lanes", type=int, default=1,
@@ -189,32 +194,71 @@ def run_c3(args):
                 "algorithmic_bytes": dk["bytes"], "avg_ms": dk["avg_ms"],
                 "copy_gbs": copy["gbs"]}
 
+    # the 32-right-hand-side batch (interleaved multivector) on the same
+    # factor: per-RHS time against one FP64 single-RHS solve
+    batch = None
+    if args.batch_cols > 0:
+        k = args.batch_cols
+        B = np.random.default_rng(11).uniform(-1.0, 1.0, (n, k))
+        log(f"batch of {k} right-hand sides (FP32 factor)")
+        s32.pcg_batch(B, args.tol, args.maxit)
+        tb = []
+        for _ in range(max(1, args.batch_steps)):
+            X, rrb, itb = s32.pcg_batch(B, args.tol, args.maxit)
+            tb.append(s32.stats()["last_solve_ms"] / 1e3)
+        worst = max(float(np.linalg.norm(Ap @ X[:, c] - B[:, c]) / np.linalg.norm(B[:, c]))
+                    for c in range(k))
+        del X
+        tbm = float(np.mean(tb))
+        batch = {"nrhs": k, "time_s": tbm, "per_rhs_s": tbm / k, "steps": len(tb),
+                 "iters_min": int(itb.min()), "iters_max": int(itb.max()),
+                 "max_true_relres": worst,
+                 "speedup_per_rhs_vs_fp64_single": solve64 / (tbm / k)}
+        log(f"batch: {tbm * 1e3:.0f} ms for {k} ({tbm / k * 1e3:.1f} ms per RHS, "
+            f"{solve64 / (tbm / k):.2f}x one FP64 solve)")
+
     cpu = None
+    parity = {}
     if not args.no_cpu:
         from oracle import oracle as O
         threads = min(16, os.cpu_count() or 1)
+        prep = O.Prepared(Ap, Gs)
+        # parity at full size: one FP64 preconditioner application and the
+        # full FP64 solve against the oracle (the solve is also the CPU
+        # baseline: sequential triangular sweeps as in the reference, OpenMP
+        # SpMV and BLAS-1)
         O.set_threads(threads)
-        log(f"CPU baseline: oracle PCG, {args.cpu_iters} iterations on {threads} threads")
+        z = s64.precond(bp)
+        zo = prep.precond(bp)
+        log(f"CPU baseline: full oracle FP64 PCG on {threads} threads")
         t = time.perf_counter()
-        O.pcg(Ap, bp, 1e-30, args.cpu_iters, Gs)
+        xo, ro, io = prep.pcg(bp, args.tol, args.maxit)
         tc = time.perf_counter() - t
-        per_it = tc / args.cpu_iters
-        cpu = {"value": per_it * it64, "unit": "s", "cores": threads, "kind": "port",
-               "sample": (f"{args.cpu_iters} FP64 PCG iterations of the same {s}^3 system "
-                          f"(oracle/pcg_oracle.c: sequential triangular sweeps, OpenMP SpMV and "
-                          f"BLAS-1 on {threads} threads), {per_it:.2f} s/iteration scaled to the "
-                          f"{it64} iterations of the FP64 solve"),
-               "s_per_iter": per_it}
-        log(f"CPU: {per_it:.2f} s/iteration -> {per_it * it64:.1f} s")
-        # the same sample on one core (the sweeps are sequential either way)
-        O.set_threads(1)
-        t = time.perf_counter()
-        O.pcg(Ap, bp, 1e-30, args.cpu_iters, Gs)
-        per_it1 = (time.perf_counter() - t) / args.cpu_iters
-        O.set_threads(threads)
-        cpu["one_core"] = {"value": per_it1 * it64, "unit": "s", "cores": 1,
-                           "s_per_iter": per_it1}
-        log(f"CPU, one core: {per_it1:.2f} s/iteration -> {per_it1 * it64:.1f} s")
+        parity = {"oracle_iters": io, "fp64_iters": it64, "mixed_iters": it32,
+                  "iters_within_1": abs(io - it64) <= 1,
+                  "fp64_x_rel_diff_vs_oracle": float(np.linalg.norm(x64 - xo) /
+                                                     np.linalg.norm(xo)),
+                  "precond_fp64_rel_diff_vs_oracle": float(np.linalg.norm(z - zo) /
+                                                           np.linalg.norm(zo)),
+                  "oracle_relres": ro}
+        log(f"CPU: {tc:.1f} s, {io} iterations (GPU FP64 {it64}); precond rel diff "
+            f"{parity['precond_fp64_rel_diff_vs_oracle']:.1e}")
+        if not parity["iters_within_1"] or parity["precond_fp64_rel_diff_vs_oracle"] > 1e-12:
+            print(f"PARITY FAILURE at full size: {parity}", file=sys.stderr, flush=True)
+        cpu = {"value": tc, "unit": "s", "cores": threads, "kind": "port",
+               "sample": (f"one full FP64 PCG solve of the same {s}^3 system to {args.tol:g} "
+                          f"(oracle/pcg_oracle.c, the C restatement of pcg.cpp: sequential "
+                          f"triangular sweeps, OpenMP SpMV and BLAS-1 on {threads} threads), "
+                          f"{io} iterations"),
+               "iters": io, "host": _host_info()}
+        if not args.no_cpu_one_core:
+            O.set_threads(1)
+            t = time.perf_counter()
+            _, _, io1 = prep.pcg(bp, args.tol, args.maxit)
+            t1 = time.perf_counter() - t
+            O.set_threads(threads)
+            cpu["one_core"] = {"value": t1, "unit": "s", "cores": 1, "iters": io1}
+            log(f"CPU, one core: {t1:.1f} s")
 
     res = {
         "metric": "PCG time-to-1e-10 & HBM GB/s, 256^3 var-coef Poisson; mixed vs FP64 GPU",
@@ -247,6 +291,8 @@ def run_c3(args):
                  "kernels": prof["fp64"], "setup_ms": setup["fp64"]},
         "speedup_mixed_vs_fp64": solve64 / solve32,
         "error": err,
+        "parity": parity,
+        "batch": batch,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -254,6 +300,25 @@ def run_c3(args):
     s32.close()
     s64.close()
     return 0
+
+
+def _host_info():
+    """nproc, CPU model and OMP_NUM_THREADS of the host the baseline ran on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {"nproc": os.cpu_count(), "affinity": affinity, "cpu_model": model,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def _emit(res):

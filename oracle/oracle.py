@@ -240,3 +240,36 @@ def stored_to_lower(Gs):
     cp, ri, v = _stored(Gs)
     n = cp.size - 1
     return sp.csc_matrix((v, ri, cp), shape=(n, n)).tocsr()
+
+
+class Prepared:
+    """A and the stored factor converted once to the reference's 64-bit
+    arrays, so that repeated oracle calls (the bench's CPU baseline) time the
+    solve and not the conversion."""
+
+    def __init__(self, A, Gs):
+        self.n, self.rp, self.ci, self.v = _csr_arrays(A)
+        self.cp, self.ri, self.gv = _stored(Gs)
+
+    def pcg(self, b, tol, maxit):
+        """pcg.cpp:57-127 from x0 = 0 -> (x, relres, itr)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n)
+        relres = ctypes.c_double()
+        itr = ctypes.c_int()
+        hist = np.empty(maxit + 1)
+        rc = lib().oracle_pcg(self.n, _p64(self.rp), _p64(self.ci), _pf(self.v), _pf(b),
+                              float(tol), int(maxit), _p64(self.cp), _p64(self.ri),
+                              _pf(self.gv), _pf(x), ctypes.byref(relres), ctypes.byref(itr),
+                              _pf(hist))
+        if rc:
+            raise ZeroDivisionError("zero pivot in G")
+        return x, relres.value, itr.value
+
+    def precond(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.empty(self.n)
+        if lib().oracle_precond(self.n, _p64(self.cp), _p64(self.ri), _pf(self.gv), _pf(r),
+                                _pf(z)):
+            raise ZeroDivisionError("zero pivot in G")
+        return z

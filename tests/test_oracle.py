@@ -227,3 +227,24 @@ def test_golden_factor_regenerates_from_reference(name):
         g = O.ref_rchol_nd(A0, perm, blocks, seed)
     for a, b in zip(g, Gs):
         np.testing.assert_array_equal(a, b)
+
+
+def test_prepared_oracle_equals_the_plain_calls():
+    """oracle.Prepared (arrays converted once, for the bench's CPU timing)
+    runs the same C code as oracle.pcg / oracle.precond: identical bits on
+    one thread (OpenMP's reduction order varies between calls)."""
+    A = O.ref_laplace_3d(6) if O.ref_available() else None
+    if A is None:
+        pytest.skip("reference build absent")
+    Gs = O.ref_rchol_seq(A, 4)
+    b = np.random.default_rng(9).uniform(-1, 1, A.shape[0])
+    P = O.Prepared(A, Gs)
+    O.set_threads(1)
+    try:
+        x1, r1, i1 = P.pcg(b, 1e-10, 100)
+        x2, r2, i2 = O.pcg(A, b, 1e-10, 100, Gs)
+    finally:
+        O.set_threads(os.cpu_count() or 1)
+    assert i1 == i2 and r1 == r2
+    np.testing.assert_array_equal(x1, x2)
+    np.testing.assert_array_equal(P.precond(b), O.precond(Gs, b))
