@@ -56,6 +56,13 @@ typedef struct rchol_amd_ctx rchol_amd_ctx;
 #define RCHOL_AMD_OPT_STRICT_SYNC 9   /* 1: the PCG loop's grid reductions publish their
                                          partials with release/acquire fences instead of
                                          value-returning atomics (same results) */
+#define RCHOL_AMD_OPT_CHUNKED 10      /* single-right-hand-side sweeps in chunks of rows held
+                                         in one workgroup (deep factors: the reference's own
+                                         orders) instead of level by level.  -1 (default):
+                                         planned at upload_G when the factor has many narrow
+                                         levels, used when its estimate beats the level
+                                         schedule; 0: never; 1: always (planned at the next
+                                         upload_G; set before it) */
 
 /* Counters filled by rchol_amd_get_stats. */
 typedef struct rchol_amd_stats {
@@ -82,6 +89,12 @@ typedef struct rchol_amd_stats {
    * sweep schedule (task cut, per-wave lists), [3] the FP32 conversion (FP32
    * factor) and the copies to the device. */
   double setup_ms[4];
+  /* 1 when the single-right-hand-side sweeps run chunked; chunks and rounds
+   * of internal entries of [0] the forward and [1] the backward plan (0: no
+   * plan). */
+  int chunked;
+  int chunks[2];
+  long long chunk_rounds[2];
 } rchol_amd_stats;
 
 /* Level schedule of the two triangular sweeps (host analysis only). */
@@ -239,6 +252,19 @@ int rchol_amd_get_stats(rchol_amd_ctx *ctx, rchol_amd_stats *out);
 int rchol_amd_analyze_G(size_t n, const size_t *colPtr, const size_t *rowIdx, const double *val,
                         int serial_width, rchol_amd_schedule *out);
 void rchol_amd_schedule_free(rchol_amd_schedule *s);
+
+/* Host-only: the plan of the chunked sweeps (RCHOL_AMD_OPT_CHUNKED) for a
+ * stored factor, chunks of at most max_rows rows, executed on the CPU in the
+ * device kernel's order of operations: z = G^-T G^-1 r.  The plan's
+ * invariants are checked on the way (a chunk reads only rows published
+ * before it, a round reads only rows no later round writes); a violation is
+ * RCHOL_AMD_EINVAL.  nchunks / rounds / est_us (may be NULL) receive the
+ * forward sweep's chunks, both sweeps' rounds and the plan's critical-path
+ * estimate of each sweep in microseconds.  For tests; no reference
+ * counterpart. */
+int rchol_amd_chunk_apply_host(size_t n, const size_t *colPtr, const size_t *rowIdx,
+                               const double *val, int max_rows, const double *r, double *z,
+                               int *nchunks, long long *rounds, double *est_us);
 
 /* Number of visible HIP devices (0 when none); never fails. */
 int rchol_amd_device_count(void);

@@ -1,0 +1,286 @@
+// Host plan of the chunked sweeps (k_sptrsv_chunk).  Included by engine.hip
+// inside its anonymous namespace, after HostSweep and par_for.
+//
+// Input: both sweeps' rows with their off-diagonal entries (HostSweep, rows
+// in level order, slot_of maps a row to its slot) and pivots.  The forward
+// sweep reads entries j < i, the backward sweep j > i.
+
+// Bounds of the chunks: one partition of [0, n) into index ranges shared by
+// both sweeps.  A range ends at S rows, and, once it holds kChunkMinRows,
+// where the next rows start a new nested-dissection block: either the next
+// kChunkWindow rows read nothing before that point (a block that depends on
+// nothing earlier: a leaf), or a row reaches before the range while the row
+// before it did not (a separator after its children).  Cuts elsewhere only
+// change the speed, never the result.
+constexpr int kChunkMinRows = 256;
+constexpr int kChunkWindow = 64;
+
+std::vector<int> chunk_bounds(const HostSweep &hf, const std::vector<int> &slot_of, int n,
+                              int S) {
+  std::vector<int> lo(n);
+  par_for((size_t)n, [&](size_t i0, size_t i1) {
+    for (size_t i = i0; i < i1; i++) {
+      const int s = slot_of[i];
+      int m = (int)i;
+      for (int e = hf.rp[s]; e < hf.rp[s + 1]; e++) m = std::min(m, hf.ci[e]);
+      lo[i] = m;
+    }
+  });
+  // wmin[e] = min(lo[e .. e + kChunkWindow)): sliding minimum, right to left
+  std::vector<int> wmin(n), dq(n);
+  {
+    int head = 0, tail = 0;  // dq[head, tail): indices, decreasing, lo increasing
+    for (int e = n - 1; e >= 0; e--) {
+      while (tail > head && lo[dq[tail - 1]] >= lo[e]) tail--;
+      dq[tail++] = e;
+      while (dq[head] >= e + kChunkWindow) head++;
+      wmin[e] = lo[dq[head]];
+    }
+  }
+  std::vector<int> b{0};
+  int c0 = 0;
+  for (int e = 1; e < n; e++) {
+    const int sz = e - c0;
+    bool cut = sz >= S;
+    if (!cut && sz >= kChunkMinRows)
+      cut = wmin[e] >= e || (lo[e] < c0 && lo[e - 1] >= c0);
+    if (cut) {
+      b.push_back(e);
+      c0 = e;
+    }
+  }
+  b.push_back(n);
+  return b;
+}
+
+struct HostChunks {
+  int nchunks = 0;
+  std::vector<int> order, q0, m, r0, r1, split;
+  std::vector<int> dep_ptr, dep;  // the chunks each chunk reads (CSR)
+  std::vector<int> q_row;
+  std::vector<double> q_diag;
+  std::vector<int> x_l, x_ci;
+  std::vector<double> x_cv;
+  std::vector<unsigned> r_w;
+  std::vector<double> r_v;
+  long long rounds = 0;
+  double est_us = 0.0;  // critical path estimate of one sweep
+};
+
+// Smallest round >= r with a free lane (union-find over full rounds).
+struct RoundFill {
+  std::vector<int> fill, next;
+  int find(int r) {
+    grow(r);
+    int a = r;
+    while (next[a] != a) {
+      a = next[a];
+      grow(a);
+    }
+    while (next[r] != a) {  // path compression
+      const int t = next[r];
+      next[r] = a;
+      r = t;
+    }
+    return a;
+  }
+  void grow(int r) {
+    while ((int)fill.size() <= r) {
+      next.push_back((int)fill.size());
+      fill.push_back(0);
+    }
+  }
+  int take(int need) {
+    const int r = find(need);
+    if (++fill[r] == 64) {
+      grow(r + 1);
+      next[r] = r + 1;
+    }
+    return r;
+  }
+};
+
+// Cost model (one sweep, microseconds): a chunk's fixed cost, a round of the
+// chain wave, a batch of external gathers per thread.  Measured orders of
+// magnitude on MI355X; only the choice between the two sweep forms uses it.
+constexpr double kChunkFixedUs = 4.0;
+constexpr double kChunkRoundUs = 0.08;
+constexpr double kChunkGatherUs = 2.0;
+
+// Plan of one sweep over the bounds b.  backward: the rows read j > i and
+// the chunks run from the last range down.
+void plan_chunks(const HostSweep &h, const std::vector<int> &slot_of, int n,
+                 const std::vector<int> &b, bool backward, HostChunks &out) {
+  const int C = (int)b.size() - 1;
+  out.nchunks = C;
+  out.q0.assign(C, 0);
+  out.m.assign(C, 0);
+  out.r0.assign(C, 0);
+  out.r1.assign(C, 0);
+  std::vector<int> lidx(n), ent_round(h.ci.size()), rounds_of(C, 0), ext_of(C, 0);
+  auto row_entries = [&](int i, int &e0, int &e1) {
+    const int s = slot_of[i];
+    e0 = h.rp[s];
+    e1 = h.rp[s + 1];
+  };
+  // pass 1: local order, local index, rounds of each internal entry
+  par_for((size_t)C, [&](size_t c0, size_t c1) {
+    std::vector<int> llev, cnt, fin, rows;
+    for (size_t c = c0; c < c1; c++) {
+      const int a = b[c], z = b[c + 1], m = z - a;
+      llev.assign(m, 0);
+      int maxlev = 0, ext = 0;
+      for (int k = 0; k < m; k++) {
+        const int i = backward ? z - 1 - k : a + k;
+        int e0, e1, lev = 0;
+        row_entries(i, e0, e1);
+        for (int e = e0; e < e1; e++) {
+          const int j = h.ci[e];
+          if (j >= a && j < z)
+            lev = std::max(lev, llev[j - a] + 1);
+          else
+            ext++;
+        }
+        llev[i - a] = lev;
+        maxlev = std::max(maxlev, lev);
+      }
+      cnt.assign(maxlev + 2, 0);
+      for (int k = 0; k < m; k++) cnt[llev[k] + 1]++;
+      for (int L = 0; L <= maxlev; L++) cnt[L + 1] += cnt[L];
+      rows.assign(m, 0);
+      for (int k = 0; k < m; k++) {
+        const int i = backward ? z - 1 - k : a + k;
+        const int l = cnt[llev[i - a]]++;
+        lidx[i] = l;
+        rows[l] = i;
+      }
+      RoundFill rf;
+      fin.assign(m, -1);
+      for (int l = 0; l < m; l++) {
+        int e0, e1;
+        row_entries(rows[l], e0, e1);
+        for (int e = e0; e < e1; e++) {
+          const int j = h.ci[e];
+          if (j < a || j >= z) continue;
+          const int r = rf.take(fin[lidx[j]] + 1);
+          ent_round[e] = r;
+          fin[l] = std::max(fin[l], r);
+        }
+      }
+      int R = 0;
+      for (int l = 0; l < m; l++) R = std::max(R, fin[l] + 1);
+      // whole prefetch groups (the chain wave's ring has no bounds test)
+      rounds_of[c] = (R + kChunkPrefetch - 1) / kChunkPrefetch * kChunkPrefetch;
+      ext_of[c] = ext;
+    }
+  });
+  // prefix sums in chunk-id order
+  std::vector<long long> xbase(C + 1, 0);
+  long long rtot = 0;
+  for (int c = 0; c < C; c++) {
+    out.q0[c] = b[c];
+    out.m[c] = b[c + 1] - b[c];
+    out.r0[c] = (int)rtot;
+    rtot += rounds_of[c];
+    out.r1[c] = (int)rtot;
+    xbase[c + 1] = xbase[c] + ext_of[c];
+  }
+  out.rounds = rtot;
+  out.q_row.assign(n, 0);
+  out.q_diag.assign(n, 0.0);
+  out.x_l.resize(xbase[C]);
+  out.x_ci.resize(xbase[C]);
+  out.x_cv.resize(xbase[C]);
+  // kChunkPrefetch rounds of padding past the last chunk: the ring's reads
+  // run that far ahead
+  out.r_w.assign((size_t)(rtot + kChunkPrefetch) * 64, kChunkPad);
+  out.r_v.assign((size_t)(rtot + kChunkPrefetch) * 64, 0.0);
+  out.split.assign((size_t)C * (kChunkThreads + 1), 0);
+  // pass 2: fill
+  par_for((size_t)C, [&](size_t c0, size_t c1) {
+    std::vector<int> lane, rows;
+    for (size_t c = c0; c < c1; c++) {
+      const int a = b[c], z = b[c + 1], m = z - a;
+      rows.assign(m, 0);
+      for (int i = a; i < z; i++) rows[lidx[i]] = i;
+      lane.assign(rounds_of[c], 0);
+      long long xp = xbase[c];
+      for (int l = 0; l < m; l++) {
+        const int i = rows[l];
+        const int s = slot_of[i];
+        const double d = h.diag[s];
+        out.q_row[a + l] = i;
+        out.q_diag[a + l] = d;
+        for (int e = h.rp[s]; e < h.rp[s + 1]; e++) {
+          const int j = h.ci[e];
+          if (j < a || j >= z) {
+            out.x_l[xp] = l;
+            out.x_ci[xp] = j;
+            out.x_cv[xp] = h.cv[e];
+            xp++;
+            continue;
+          }
+          const int r = ent_round[e];
+          const size_t slot = ((size_t)out.r0[c] + r) * 64 + lane[r]++;
+          out.r_w[slot] = (unsigned)lidx[j] | ((unsigned)l << 16);
+          out.r_v[slot] = h.cv[e] / d;
+        }
+      }
+      // a row-aligned, balanced split of the external entries over the threads
+      const long long x0 = xbase[c], x1 = xbase[c + 1];
+      int *sp = &out.split[c * (size_t)(kChunkThreads + 1)];
+      long long k = x0;
+      for (int t = 0; t <= kChunkThreads; t++) {
+        long long target = x0 + (x1 - x0) * t / kChunkThreads;
+        if (target < k) target = k;
+        while (target < x1 && target > x0 && out.x_l[target] == out.x_l[target - 1]) target++;
+        sp[t] = (int)target;
+        k = target;
+      }
+      sp[kChunkThreads] = (int)x1;
+    }
+  });
+  // dequeue order: by depth in the chunk graph, then in sweep order; and the
+  // critical path estimate
+  std::vector<int> chunk_of(n);
+  for (int c = 0; c < C; c++)
+    for (int i = b[c]; i < b[c + 1]; i++) chunk_of[i] = c;
+  std::vector<int> depth(C, 0);
+  std::vector<double> finish(C, 0.0);
+  std::vector<std::vector<int>> deps(C);
+  par_for((size_t)C, [&](size_t c0, size_t c1) {
+    std::vector<char> seen;
+    for (size_t c = c0; c < c1; c++) {
+      std::vector<int> &d = deps[c];
+      for (long long e = xbase[c]; e < xbase[c + 1]; e++) d.push_back(chunk_of[out.x_ci[e]]);
+      std::sort(d.begin(), d.end());
+      d.erase(std::unique(d.begin(), d.end()), d.end());
+    }
+  });
+  out.dep_ptr.assign(C + 1, 0);
+  for (int c = 0; c < C; c++) out.dep_ptr[c + 1] = out.dep_ptr[c] + (int)deps[c].size();
+  out.dep.resize(out.dep_ptr[C]);
+  for (int c = 0; c < C; c++) std::copy(deps[c].begin(), deps[c].end(), out.dep.begin() + out.dep_ptr[c]);
+  double crit = 0.0;
+  for (int t = 0; t < C; t++) {
+    const int c = backward ? C - 1 - t : t;
+    int dep = 0;
+    double start = 0.0;
+    for (int p : deps[c]) {
+      dep = std::max(dep, depth[p] + 1);
+      start = std::max(start, finish[p]);
+    }
+    depth[c] = dep;
+    const double batches =
+        std::ceil((double)(xbase[c + 1] - xbase[c]) / (kChunkThreads * kChunkExtBatch));
+    finish[c] = start + kChunkFixedUs + rounds_of[c] * kChunkRoundUs + batches * kChunkGatherUs;
+    crit = std::max(crit, finish[c]);
+  }
+  out.est_us = crit;
+  out.order.resize(C);
+  for (int c = 0; c < C; c++) out.order[c] = c;
+  std::stable_sort(out.order.begin(), out.order.end(), [&](int p, int q) {
+    if (depth[p] != depth[q]) return depth[p] < depth[q];
+    return backward ? p > q : p < q;
+  });
+}

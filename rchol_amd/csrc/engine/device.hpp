@@ -31,7 +31,10 @@ struct DevScalars {
   unsigned int ctr[8];  // last-block counters of the reductions (self-resetting)
 };
 
-enum { CTR_INIT = 0, CTR_BWD = 1, CTR_SPMVP = 2, CTR_UPD = 3, CTR_RES = 4, CTR_GEN = 5 };
+// ctr[CTR_CHUNK_Q] / [CTR_CHUNK_Q + 1]: chunk queue and exit count of the
+// chunked sweeps (self-resetting)
+enum { CTR_INIT = 0, CTR_BWD = 1, CTR_SPMVP = 2, CTR_UPD = 3, CTR_RES = 4, CTR_GEN = 5,
+       CTR_CHUNK_Q = 6 };
 
 // Sentinel bit patterns: signalling NaNs that arithmetic never produces.
 constexpr unsigned long long kSent64 = 0x7FF4DEADBEEFCAFEull;

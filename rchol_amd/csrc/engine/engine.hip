@@ -26,6 +26,7 @@
 #include "kernels.hpp"
 #include "kernels_mr.hpp"
 #include "kernels_seg.hpp"
+#include "kernels_chunk.hpp"
 
 using namespace rchol_amd;
 
@@ -124,6 +125,32 @@ struct Sweep {
   int head_end = 0, tail_begin = 0, mid_ntasks = 0;
   DevBuf<double> cv64, diag64;
   DevBuf<float> cv32, diag32;
+  // chunked form (k_sptrsv_chunk), built when the factor's chains are deep
+  struct Chunks {
+    int nchunks = 0;
+    long long rounds = 0;
+    double est_us = 0.0;
+    DevBuf<int> order, q0, m, r0, r1, split, q_row, x_l, x_ci, dep_ptr, dep;
+    DevBuf<int> done;  // per chunk: published (reset by each launch's last workgroup)
+    DevBuf<unsigned> r_w;
+    DevBuf<double> q_diag64, x_cv64, r_v64;
+    DevBuf<float> q_diag32, x_cv32, r_v32;
+    void reset() {
+      nchunks = 0;
+      rounds = 0;
+      est_us = 0.0;
+      for (DevBuf<int> *b : {&order, &q0, &m, &r0, &r1, &split, &q_row, &x_l, &x_ci, &dep_ptr, &dep,
+                             &done})
+        b->reset();
+      r_w.reset();
+      q_diag64.reset();
+      x_cv64.reset();
+      r_v64.reset();
+      q_diag32.reset();
+      x_cv32.reset();
+      r_v32.reset();
+    }
+  } ch;
 };
 
 // Host-side schedule of one sweep before upload.
@@ -265,6 +292,8 @@ void order_entries_by_level(HostSweep &hs, const std::vector<int> &lev) {
     }
   });
 }
+
+#include "chunk_plan.hpp"
 
 // Cut a sweep's levels into a narrow head, a middle and a narrow tail
 // (levels of at most `width` rows run in one workgroup; width 0 = no split).
@@ -492,6 +521,12 @@ struct rchol_amd_ctx {
   int coherent_gather = 0;  // sweeps read x with agent-scope loads from the start
   int task_waves = 0;       // waves the task cut plans for (0 = resident capacity)
   int strict_sync = 0;      // grid reductions with release/acquire fences
+  int chunk_mode = -1;      // RCHOL_AMD_OPT_CHUNKED
+  bool chunk_auto = false;  // the upload's own choice (chunk_mode -1)
+  bool use_chunks = false;  // single-RHS sweeps run k_sptrsv_chunk
+  int lds_max = 0;          // LDS bytes a workgroup may allocate
+  int chunk_grid = 0;       // workgroups of the chunked sweeps
+  size_t chunk_lds = 0;     // their dynamic LDS bytes
   long long g_gen = 0;      // bumped by every upload_G (keys the batched schedule)
   std::vector<hipEvent_t> pev;  // 5 per captured iteration when profiling
   // captured iteration graph
@@ -609,6 +644,20 @@ template <typename VT, typename XT, typename BT, int FUSE>
 void launch_one_sweep(rchol_amd_ctx *c, const Sweep &sw, const VT *cv, const VT *dg,
                       const BT *rhs, XT *x, XT *aux, const double *dotv, const int *stop,
                       DevScalars *sc) {
+  if (c->use_chunks && sw.ch.nchunks > 0) {
+    const Sweep::Chunks &h = sw.ch;
+    const VT *qd, *xcv, *rv;
+    if constexpr (sizeof(VT) == 8) {
+      qd = h.q_diag64.p; xcv = h.x_cv64.p; rv = h.r_v64.p;
+    } else {
+      qd = h.q_diag32.p; xcv = h.x_cv32.p; rv = h.r_v32.p;
+    }
+    hipLaunchKernelGGL((k_sptrsv_chunk<VT, XT, BT, FUSE>), dim3(c->chunk_grid), dim3(kChunkThreads),
+                       c->chunk_lds, c->stream, h.nchunks, h.order.p, h.q0.p, h.m.p, h.r0.p,
+                       h.r1.p, h.split.p, h.dep_ptr.p, h.dep.p, h.done.p, h.q_row.p, qd, h.x_l.p,
+                       h.x_ci.p, xcv, h.r_w.p, rv, rhs, x, aux, dotv, c->partials.p, sc, stop);
+    return;
+  }
   const bool head = sw.head_end > 0, tail = sw.tail_begin < sw.nlevels;
   if (head)
     hipLaunchKernelGGL((k_sptrsv_serial<VT, XT, BT, FUSE>), dim3(1), dim3(1024), 0, c->stream, 0,
@@ -1518,6 +1567,7 @@ int rchol_amd_create(int device, rchol_amd_ctx **out) {
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, device));
   c->ncu = prop.multiProcessorCount;
+  c->lds_max = (int)prop.sharedMemPerBlock;
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIP_TRY(hipMalloc((void **)&c->sc, sizeof(DevScalars)));
   HIP_TRY(hipMemset(c->sc, 0, sizeof(DevScalars)));
@@ -1752,6 +1802,130 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
   return RCHOL_AMD_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// A factor is "deep" when its levels are many and narrow on average: then
+// the chunked sweeps are planned and their estimate compared with the level
+// schedule's (about kLevelUs per level, measured on MI355X).
+constexpr int kChunkMinLevels = 1024;
+constexpr double kChunkMaxMeanWidth = 4096.0;
+constexpr double kLevelUs = 1.3;
+
+template <typename T>
+hipError_t upload_as(DevBuf<T> &d, const std::vector<double> &h) {
+  if constexpr (sizeof(T) == sizeof(double)) {
+    return d.upload((const T *)h.data(), h.size());
+  } else {
+    std::vector<T, NoInit<T>> f(h.size());
+    par_for(h.size(), [&](size_t e0, size_t e1) {
+      for (size_t e = e0; e < e1; e++) f[e] = (T)h[e];
+    });
+    return d.upload(f.data(), f.size());
+  }
+}
+
+int upload_chunks(Sweep::Chunks &d, const HostChunks &h, int precision) {
+  d.reset();
+  HIP_TRY(d.order.upload(h.order.data(), h.order.size()));
+  HIP_TRY(d.q0.upload(h.q0.data(), h.q0.size()));
+  HIP_TRY(d.m.upload(h.m.data(), h.m.size()));
+  HIP_TRY(d.r0.upload(h.r0.data(), h.r0.size()));
+  HIP_TRY(d.r1.upload(h.r1.data(), h.r1.size()));
+  HIP_TRY(d.split.upload(h.split.data(), h.split.size()));
+  HIP_TRY(d.q_row.upload(h.q_row.data(), h.q_row.size()));
+  HIP_TRY(d.x_l.upload(h.x_l.data(), h.x_l.size()));
+  HIP_TRY(d.x_ci.upload(h.x_ci.data(), h.x_ci.size()));
+  HIP_TRY(d.r_w.upload(h.r_w.data(), h.r_w.size()));
+  HIP_TRY(d.dep_ptr.upload(h.dep_ptr.data(), h.dep_ptr.size()));
+  HIP_TRY(d.dep.upload(h.dep.data(), h.dep.size()));
+  HIP_TRY(d.done.alloc(h.nchunks));
+  HIP_TRY(hipMemset(d.done.p, 0, std::max(h.nchunks, 1) * sizeof(int)));
+  if (precision == RCHOL_AMD_FP32) {
+    HIP_TRY(upload_as(d.q_diag32, h.q_diag));
+    HIP_TRY(upload_as(d.x_cv32, h.x_cv));
+    HIP_TRY(upload_as(d.r_v32, h.r_v));
+  } else {
+    HIP_TRY(upload_as(d.q_diag64, h.q_diag));
+    HIP_TRY(upload_as(d.x_cv64, h.x_cv));
+    HIP_TRY(upload_as(d.r_v64, h.r_v));
+  }
+  d.nchunks = h.nchunks;
+  d.rounds = h.rounds;
+  d.est_us = h.est_us;
+  return RCHOL_AMD_OK;
+}
+
+// Dynamic LDS and grid of the chunked sweeps of one precision: as many
+// workgroups as fit the CUs (any grid completes: the chunk queue needs no
+// co-residency).
+template <typename VT, typename XT>
+int chunk_launch_setup(rchol_amd_ctx *c, size_t lds) {
+  const void *fns[3] = {(const void *)k_sptrsv_chunk<VT, XT, double, 1>,
+                        (const void *)k_sptrsv_chunk<VT, XT, XT, 2>,
+                        (const void *)k_sptrsv_chunk<VT, XT, XT, 0>};
+  int per_cu = 1 << 20;
+  for (const void *f : fns) {
+    HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int b = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, f, kChunkThreads, lds));
+    per_cu = std::min(per_cu, b);
+  }
+  if (per_cu < 1)
+    return fail(RCHOL_AMD_EHIP, "the chunked sweep does not fit a CU with %zu bytes of LDS", lds);
+  c->chunk_grid = per_cu * c->ncu;
+  c->chunk_lds = lds;
+  return RCHOL_AMD_OK;
+}
+
+// Plan (and, when chosen, upload) the chunked sweeps of a factor whose
+// analysis is in hf / hb.
+int setup_chunks(rchol_amd_ctx *c, size_t n, const HostSweep &hf, const HostSweep &hb, int nlf,
+                 int nlb, int precision) {
+  c->fwd.ch.reset();
+  c->bwd.ch.reset();
+  c->use_chunks = false;
+  c->chunk_auto = false;
+  const int nl = std::max(nlf, nlb);
+  const bool deep = nl >= kChunkMinLevels && (double)n / nl < kChunkMaxMeanWidth;
+  if (!(c->chunk_mode == 1 || (c->chunk_mode == -1 && deep))) return RCHOL_AMD_OK;
+  const size_t xt = precision == RCHOL_AMD_FP32 ? sizeof(float) : sizeof(double);
+  const size_t budget = c->lds_max > 2048 ? (size_t)c->lds_max - 1024 : 0;
+  const int S = (int)std::min<size_t>(65535, budget / xt - 1);
+  if (S < kChunkMinRows)
+    return c->chunk_mode == 1 ? fail(RCHOL_AMD_EHIP, "%d bytes of LDS per workgroup is too little "
+                                     "for the chunked sweeps", c->lds_max)
+                              : RCHOL_AMD_OK;
+  std::vector<int> slot_of_f(n), slot_of_b(n);
+  for (size_t s = 0; s < n; s++) {
+    slot_of_f[hf.slot_row[s]] = (int)s;
+    slot_of_b[hb.slot_row[s]] = (int)s;
+  }
+  const std::vector<int> bnd = chunk_bounds(hf, slot_of_f, (int)n, S);
+  HostChunks pf, pb;
+  plan_chunks(hf, slot_of_f, (int)n, bnd, false, pf);
+  plan_chunks(hb, slot_of_b, (int)n, bnd, true, pb);
+  c->chunk_auto = pf.est_us + pb.est_us < (nlf + nlb) * kLevelUs;
+  if (c->chunk_mode == -1 && !c->chunk_auto) return RCHOL_AMD_OK;
+  int rc = upload_chunks(c->fwd.ch, pf, precision);
+  if (!rc) rc = upload_chunks(c->bwd.ch, pb, precision);
+  if (!rc)
+    rc = precision == RCHOL_AMD_FP32 ? chunk_launch_setup<float, float>(c, (S + 1) * xt)
+                                     : chunk_launch_setup<double, double>(c, (S + 1) * xt);
+  if (rc) {
+    c->fwd.ch.reset();
+    c->bwd.ch.reset();
+    return rc;
+  }
+  c->use_chunks = true;
+  return RCHOL_AMD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int rchol_amd_upload_G(rchol_amd_ctx *c, size_t n, const size_t *colPtr, const size_t *rowIdx,
                        const double *val, int precision) {
   if (!c || !colPtr || !rowIdx || !val) return fail(RCHOL_AMD_EINVAL, "null argument");
@@ -1813,7 +1987,14 @@ int rchol_amd_upload_G(rchol_amd_ctx *c, size_t n, const size_t *colPtr, const s
   const double t2 = host_ms();
   rc = c->size_trsv_grid();
   if (rc) return rc;
+  rc = setup_chunks(c, n, hf, hb, nlf, nlb, precision);
+  if (rc) return rc;
   c->hasG = true;
+  c->stats.chunked = c->use_chunks ? 1 : 0;
+  c->stats.chunks[0] = c->fwd.ch.nchunks;
+  c->stats.chunks[1] = c->bwd.ch.nchunks;
+  c->stats.chunk_rounds[0] = c->fwd.ch.rounds;
+  c->stats.chunk_rounds[1] = c->bwd.ch.rounds;
   c->stats.setup_ms[1] = t1 - t0;
   c->stats.setup_ms[2] = host_ms() - t2;
   c->stats.setup_ms[3] = t2 - t1;
@@ -1854,6 +2035,18 @@ int rchol_amd_set_option(rchol_amd_ctx *c, int key, double value) {
       break;
     case RCHOL_AMD_OPT_STRICT_SYNC:
       c->strict_sync = v ? 1 : 0;
+      break;
+    case RCHOL_AMD_OPT_CHUNKED:
+      if (v < -1 || v > 1) return fail(RCHOL_AMD_EINVAL, "chunked mode %d not in {-1, 0, 1}", v);
+      if (c->hasG && v == 1 && c->fwd.ch.nchunks == 0)
+        return fail(RCHOL_AMD_ESTATE,
+                    "this factor has no chunked plan: set the option to 1 before upload_G");
+      c->chunk_mode = v;
+      if (c->hasG) {
+        c->use_chunks = c->fwd.ch.nchunks > 0 && (v == 1 || (v == -1 && c->chunk_auto));
+        c->stats.chunked = c->use_chunks ? 1 : 0;
+        c->drop_graph();
+      }
       break;
     case RCHOL_AMD_OPT_TASK_WAVES:
       if (v < 0) return fail(RCHOL_AMD_EINVAL, "negative task wave count");
@@ -2344,6 +2537,125 @@ int rchol_amd_analyze_G(size_t n, const size_t *colPtr, const size_t *rowIdx, co
     if (!out->level_ptr[k] || !out->slot_row[k]) return fail(RCHOL_AMD_EINVAL, "out of memory");
     memcpy(out->level_ptr[k], h[k].level_ptr.data(), (nl[k] + 1) * sizeof(int));
     memcpy(out->slot_row[k], h[k].slot_row.data(), n * sizeof(int));
+  }
+  return RCHOL_AMD_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// The chunked plan of one sweep run on the host in the kernel's order of
+// operations (double arithmetic), checking its invariants: a chunk reads
+// only rows of chunks dequeued before it, every thread's run of external
+// entries starts on a row boundary, a round reads only rows that no later
+// round writes.
+int simulate_chunks(const HostChunks &p, int n, const double *rhs, double *out) {
+  std::vector<char> done(n, 0), listed(n, 0);
+  std::vector<double> yl;
+  std::vector<int> last;
+  for (int k = 0; k < p.nchunks; k++) {
+    const int c = p.order[k];
+    const int q0 = p.q0[c], m = p.m[c];
+    yl.assign(m, 0.0);
+    for (int l = 0; l < m; l++) yl[l] = rhs[p.q_row[q0 + l]];
+    // rows of the chunks this one waits for (cleared again below)
+    auto mark = [&](char v) {
+      for (int d = p.dep_ptr[c]; d < p.dep_ptr[c + 1]; d++) {
+        const int cd = p.dep[d];
+        for (int l = 0; l < p.m[cd]; l++) listed[p.q_row[p.q0[cd] + l]] = v;
+      }
+    };
+    mark(1);
+    const int *sp = &p.split[(size_t)c * (kChunkThreads + 1)];
+    for (int t = 0; t < kChunkThreads; t++) {
+      if (sp[t] > sp[t + 1]) return fail(RCHOL_AMD_EINVAL, "chunk %d: split not monotone", c);
+      if (t > 0 && sp[t] > sp[0] && sp[t] < sp[kChunkThreads] && p.x_l[sp[t]] == p.x_l[sp[t] - 1])
+        return fail(RCHOL_AMD_EINVAL, "chunk %d: thread %d starts inside a row", c, t);
+      int cur = -1;
+      double acc = 0.0;
+      for (int e = sp[t]; e < sp[t + 1]; e++) {
+        const int j = p.x_ci[e];
+        if (!done[j]) return fail(RCHOL_AMD_EINVAL, "chunk %d reads row %d before it is published", c, j);
+        if (!listed[j])
+          return fail(RCHOL_AMD_EINVAL, "chunk %d reads row %d of a chunk it does not wait for", c, j);
+        if (p.x_l[e] != cur) {
+          if (cur >= 0) yl[cur] -= acc;
+          cur = p.x_l[e];
+          acc = 0.0;
+        }
+        acc += p.x_cv[e] * out[j];
+      }
+      if (cur >= 0) yl[cur] -= acc;
+    }
+    mark(0);
+    for (int l = 0; l < m; l++) yl[l] = yl[l] / p.q_diag[q0 + l];
+    last.assign(m, -1);
+    for (int r = p.r0[c]; r < p.r1[c]; r++)
+      for (int ln = 0; ln < 64; ln++) {
+        const unsigned w = p.r_w[(size_t)r * 64 + ln];
+        if (w == kChunkPad) continue;
+        if ((int)(w >> 16) >= m || (int)(w & 0xFFFFu) >= m)
+          return fail(RCHOL_AMD_EINVAL, "chunk %d: round %d names a row outside the chunk", c, r);
+        last[w >> 16] = r;
+      }
+    for (int r = p.r0[c]; r < p.r1[c]; r++)
+      for (int ln = 0; ln < 64; ln++) {
+        const unsigned w = p.r_w[(size_t)r * 64 + ln];
+        if (w == kChunkPad) continue;
+        const unsigned src = w & 0xFFFFu, tgt = w >> 16;
+        if (last[src] >= r)
+          return fail(RCHOL_AMD_EINVAL, "chunk %d: round %d reads row %u, written in round %d", c,
+                      r, src, last[src]);
+        yl[tgt] -= p.r_v[(size_t)r * 64 + ln] * yl[src];
+      }
+    for (int l = 0; l < m; l++) {
+      const int row = p.q_row[q0 + l];
+      if (done[row]) return fail(RCHOL_AMD_EINVAL, "row %d belongs to two chunks", row);
+      out[row] = yl[l];
+      done[row] = 1;
+    }
+  }
+  for (int i = 0; i < n; i++)
+    if (!done[i]) return fail(RCHOL_AMD_EINVAL, "row %d is in no chunk", i);
+  return RCHOL_AMD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rchol_amd_chunk_apply_host(size_t n, const size_t *colPtr, const size_t *rowIdx,
+                               const double *val, int max_rows, const double *r, double *z,
+                               int *nchunks, long long *rounds, double *est_us) {
+  if (!colPtr || !rowIdx || !val || !r || !z) return fail(RCHOL_AMD_EINVAL, "null argument");
+  if (n == 0 || n > kIdxMax) return fail(RCHOL_AMD_ERANGE, "n=%zu outside [1, 2^31)", n);
+  if (colPtr[0] != 0) return fail(RCHOL_AMD_EINVAL, "colPtr[0] must be 0");
+  if (colPtr[n] > kIdxMax) return fail(RCHOL_AMD_ERANGE, "nnz(G) exceeds the int32 index range");
+  if (max_rows < 1 || max_rows > 65535)
+    return fail(RCHOL_AMD_EINVAL, "chunk rows %d not in [1, 65535]", max_rows);
+  HostSweep hf, hb;
+  int nlf = 0, nlb = 0;
+  int rc = analyze_factor(n, colPtr, rowIdx, val, hf, hb, nlf, nlb);
+  if (rc) return rc;
+  std::vector<int> slot_of_f(n), slot_of_b(n);
+  for (size_t s = 0; s < n; s++) {
+    slot_of_f[hf.slot_row[s]] = (int)s;
+    slot_of_b[hb.slot_row[s]] = (int)s;
+  }
+  const std::vector<int> bnd = chunk_bounds(hf, slot_of_f, (int)n, max_rows);
+  HostChunks pf, pb;
+  plan_chunks(hf, slot_of_f, (int)n, bnd, false, pf);
+  plan_chunks(hb, slot_of_b, (int)n, bnd, true, pb);
+  std::vector<double> y(n);
+  rc = simulate_chunks(pf, (int)n, r, y.data());
+  if (!rc) rc = simulate_chunks(pb, (int)n, y.data(), z);
+  if (rc) return rc;
+  if (nchunks) *nchunks = pf.nchunks;
+  if (rounds) *rounds = pf.rounds + pb.rounds;
+  if (est_us) {
+    est_us[0] = pf.est_us;
+    est_us[1] = pb.est_us;
   }
   return RCHOL_AMD_OK;
 }

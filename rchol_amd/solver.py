@@ -24,6 +24,7 @@ OPT_SERIAL_WIDTH = 6
 OPT_COHERENT_GATHER = 7
 OPT_TASK_WAVES = 8
 OPT_STRICT_SYNC = 9
+OPT_CHUNKED = 10
 
 
 class Stats(ctypes.Structure):
@@ -43,12 +44,17 @@ class Stats(ctypes.Structure):
         ("kernel_ms", ctypes.c_double * 4),
         ("kernel_iters", ctypes.c_int),
         ("setup_ms", ctypes.c_double * 4),
+        ("chunked", ctypes.c_int),
+        ("chunks", ctypes.c_int * 2),
+        ("chunk_rounds", ctypes.c_longlong * 2),
     ]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
         d["kernel_ms"] = list(self.kernel_ms)
         d["setup_ms"] = list(self.setup_ms)
+        d["chunks"] = list(self.chunks)
+        d["chunk_rounds"] = list(self.chunk_rounds)
         return d
 
 
@@ -92,7 +98,7 @@ class Solver:
     """
 
     def __init__(self, A=None, G=None, precision=FP64, device=0, graph_iters=None,
-                 spmv_lanes=None, trsv_lanes=None):
+                 spmv_lanes=None, trsv_lanes=None, chunked=None):
         self._lib = _lib.engine()
         h = ctypes.c_void_p()
         _lib.check(self._lib.rchol_amd_create(int(device), ctypes.byref(h)))
@@ -104,6 +110,8 @@ class Solver:
             self.set_option(OPT_SPMV_LANES, spmv_lanes)
         if trsv_lanes is not None:
             self.set_option(OPT_TRSV_LANES, trsv_lanes)
+        if chunked is not None:
+            self.set_option(OPT_CHUNKED, chunked)
         if A is not None:
             self.upload_A(A)
         if G is not None:
@@ -357,6 +365,31 @@ def analyze(G, serial_width=0):
         }
     lib.rchol_amd_schedule_free(ctypes.byref(sc))
     return out
+
+
+def chunk_apply_host(G, r, max_rows=65535):
+    """The chunked sweeps' plan for G (chunks of at most max_rows rows) run
+    on the host in the device kernel's order of operations: z = G^-T G^-1 r.
+    Raises on a plan that breaks its invariants.  Returns (z, forward chunks,
+    rounds of both sweeps, critical-path estimate of [forward, backward] in
+    microseconds)."""
+    if isinstance(G, tuple):
+        cp, ri, gv = G
+    else:
+        cp, ri, gv = stored_from_lower(G)
+    cp, pcp = _u64(cp)
+    ri, pri = _u64(ri)
+    gv, pgv = _f64(gv)
+    r, pr = _f64(r)
+    z = np.empty_like(r)
+    nch = ctypes.c_int()
+    nrd = ctypes.c_longlong()
+    est = np.zeros(2)
+    lib = _lib.engine()
+    _lib.check(lib.rchol_amd_chunk_apply_host(cp.size - 1, pcp, pri, pgv, int(max_rows), pr,
+                                              z.ctypes.data_as(_lib.f64p), ctypes.byref(nch),
+                                              ctypes.byref(nrd), est.ctypes.data_as(_lib.f64p)))
+    return z, nch.value, nrd.value, est
 
 
 def pcg(A, b, epsilon, maxitr, G, Gt=None):

@@ -52,8 +52,10 @@ def check_solve(A, Gs, b, tag, maxit=5000):
     with rchol_amd.Solver(A, Gs, rchol_amd.FP64) as s:
         z = s.precond(b)
         x, relres, itr = s.pcg(b, TOL, maxit)
+        st = s.stats()
     zo = O.precond(Gs, b)
-    print(f"{tag}: itr {itr} oracle {io}; precond {rel(z, zo):.1e}; x {rel(x, xo):.1e}",
+    print(f"{tag}: itr {itr} oracle {io}; precond {rel(z, zo):.1e}; x {rel(x, xo):.1e}; "
+          f"chunked {st['chunked']} ({st['chunks']} chunks), {st['last_solve_ms']:.1f} ms",
           flush=True)
     assert rel(z, zo) <= PRECOND_RTOL
     assert abs(itr - io) <= 1

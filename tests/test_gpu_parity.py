@@ -327,3 +327,26 @@ def test_option_values_are_validated():
         x, relres, itr = s.pcg(d["b"], 1e-10, 5000)
     assert rel(z, d["z"]) <= PRECOND_RTOL
     assert abs(itr - int(d["itr"])) <= 1
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+@pytest.mark.parametrize("precision", [rchol_amd.FP64, rchol_amd.FP32])
+def test_chunked_sweeps_match_oracle(name, precision):
+    """The chunked sweeps (forced on these small factors) against the
+    oracle: application, repeatability, PCG iterations and solution."""
+    A, Gs, d = load(name)
+    print(f"case chunked {name} prec={precision}", flush=True)
+    with rchol_amd.Solver(A, Gs, precision, chunked=1) as s:
+        assert s.stats()["chunked"] == 1
+        z = s.precond(d["b"])
+        np.testing.assert_array_equal(z, s.precond(d["b"]))
+        x, relres, itr = s.pcg(d["b"], float(d["tol"]), 5000)
+    if precision == rchol_amd.FP64:
+        assert rel(z, d["z"]) <= PRECOND_RTOL
+        assert abs(itr - int(d["itr"])) <= 1
+        assert rel(x, d["x"]) <= 1e-8
+    else:
+        assert rel(z, d["z"]) <= 1e-5
+    true = np.linalg.norm(A @ x - d["b"]) / np.linalg.norm(d["b"])
+    assert true <= float(d["tol"]) * 10
+    assert relres == pytest.approx(true, rel=1e-6)
