@@ -258,13 +258,14 @@ void rchol_amd_schedule_free(rchol_amd_schedule *s);
  * device kernel's order of operations: z = G^-T G^-1 r.  The plan's
  * invariants are checked on the way (a chunk reads only rows published
  * before it, a round reads only rows no later round writes); a violation is
- * RCHOL_AMD_EINVAL.  nchunks / rounds / est_us (may be NULL) receive the
- * forward sweep's chunks, both sweeps' rounds and the plan's critical-path
- * estimate of each sweep in microseconds.  For tests; no reference
+ * RCHOL_AMD_EINVAL.  counts[3] / rounds / est_us[2] (may be NULL) receive
+ * the chunks (both sweeps share them) and the external parts of the forward
+ * and of the backward plan, both sweeps' rounds, and the plans'
+ * critical-path estimates in microseconds.  For tests; no reference
  * counterpart. */
 int rchol_amd_chunk_apply_host(size_t n, const size_t *colPtr, const size_t *rowIdx,
                                const double *val, int max_rows, const double *r, double *z,
-                               int *nchunks, long long *rounds, double *est_us);
+                               int *counts, long long *rounds, double *est_us);
 
 /* Number of visible HIP devices (0 when none); never fails. */
 int rchol_amd_device_count(void);

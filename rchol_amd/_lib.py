@@ -40,7 +40,7 @@ ENGINE_SYMBOLS = {
     "rchol_amd_precond_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                                 ctypes.c_void_p]),
     "rchol_amd_chunk_apply_host": (ctypes.c_int, [ctypes.c_size_t, u64p, u64p, f64p, ctypes.c_int,
-                                                  f64p, f64p, ctypes.POINTER(ctypes.c_int),
+                                                  f64p, f64p, i32p,
                                                   ctypes.POINTER(ctypes.c_longlong), f64p]),
     "rchol_amd_precond_device_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                                 ctypes.c_void_p]),

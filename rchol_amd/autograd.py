@@ -40,6 +40,8 @@ class GridLaplacian:
         dev = torch.device("cuda", device)
         self.u = torch.as_tensor(u, dtype=torch.int64, device=dev)
         self.v = torch.as_tensor(v, dtype=torch.int64, device=dev)
+        # the nested dissection depends on the grid only: computed once
+        self._perm, self._blocks = F.nd_grid(*self.dims, self.depth)
         self._key = None
         self._solver = None
         self._w = None  # device copy of the weights the cached factor belongs to
@@ -59,10 +61,8 @@ class GridLaplacian:
         key = hashlib.blake2b(w.tobytes(), digest_size=32).digest()
         if key != self._key:
             A = self.matrix(w)
-            perm, blocks = F.nd_grid(*self.dims, self.depth)
-            Gs, po, _ = F.factor(A, perm, blocks, self.order, self.seed)
-            P = po.astype(np.int64)
-            Ap = A[P][:, P].tocsr()
+            Gs, po, _ = F.factor(A, self._perm, self._blocks, self.order, self.seed)
+            Ap = F.permute(A, po)
             s = Solver(Ap, Gs, self.precision, device=self.device)
             s.set_perm(po)
             self._solver = s

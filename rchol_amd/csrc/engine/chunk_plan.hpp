@@ -53,10 +53,16 @@ std::vector<int> chunk_bounds(const HostSweep &hf, const std::vector<int> &slot_
   return b;
 }
 
+// External entries per part (see plan_chunks); a chunk with more than two
+// parts' worth hands its external sums to parts.
+constexpr long long kExtPartEntries = 8192;
+
 struct HostChunks {
-  int nchunks = 0;
+  int nchunks = 0, nparts = 0;
+  std::vector<int> c_part0, p_chunk, p_l0, p_l1, p_lg;  // parts of the external sums
+  std::vector<int> x_rp;  // per chunk-local row (q): its first external entry
   std::vector<int> order, q0, m, r0, r1, split;
-  std::vector<int> dep_ptr, dep;  // the chunks each chunk reads (CSR)
+  std::vector<int> dep_ptr, dep;  // the work items each work item waits for (CSR)
   std::vector<int> q_row;
   std::vector<double> q_diag;
   std::vector<int> x_l, x_ci;
@@ -188,6 +194,7 @@ void plan_chunks(const HostSweep &h, const std::vector<int> &slot_of, int n,
   out.rounds = rtot;
   out.q_row.assign(n, 0);
   out.q_diag.assign(n, 0.0);
+  out.x_rp.assign(n + 1, 0);
   out.x_l.resize(xbase[C]);
   out.x_ci.resize(xbase[C]);
   out.x_cv.resize(xbase[C]);
@@ -196,6 +203,7 @@ void plan_chunks(const HostSweep &h, const std::vector<int> &slot_of, int n,
   out.r_w.assign((size_t)(rtot + kChunkPrefetch) * 64, kChunkPad);
   out.r_v.assign((size_t)(rtot + kChunkPrefetch) * 64, 0.0);
   out.split.assign((size_t)C * (kChunkThreads + 1), 0);
+  out.x_rp[n] = (int)xbase[C];
   // pass 2: fill
   par_for((size_t)C, [&](size_t c0, size_t c1) {
     std::vector<int> lane, rows;
@@ -211,6 +219,7 @@ void plan_chunks(const HostSweep &h, const std::vector<int> &slot_of, int n,
         const double d = h.diag[s];
         out.q_row[a + l] = i;
         out.q_diag[a + l] = d;
+        out.x_rp[a + l] = (int)xp;
         for (int e = h.rp[s]; e < h.rp[s + 1]; e++) {
           const int j = h.ci[e];
           if (j < a || j >= z) {
@@ -240,8 +249,65 @@ void plan_chunks(const HostSweep &h, const std::vector<int> &slot_of, int n,
       sp[kChunkThreads] = (int)x1;
     }
   });
-  // dequeue order: by depth in the chunk graph, then in sweep order; and the
-  // critical path estimate
+  // external parts: a chunk with many external entries hands their sums to
+  // parts, separate work items that other workgroups run in parallel, each
+  // over a row-aligned range of the chunk's local rows (it writes the sum of
+  // every row of its range, zero for rows without entries, with row groups
+  // of 2^p_lg lanes); the chunk then starts from those sums
+  std::vector<int> part0(C + 1, 0);
+  for (int c = 0; c < C; c++) {
+    const long long E = xbase[c + 1] - xbase[c];
+    part0[c + 1] = part0[c] + (E > 2 * kExtPartEntries
+                                   ? (int)((E + kExtPartEntries - 1) / kExtPartEntries)
+                                   : 0);
+  }
+  const int P = part0[C];
+  out.nparts = P;
+  out.c_part0 = part0;
+  out.p_chunk.assign(P, 0);
+  out.p_l0.assign(P, 0);
+  out.p_l1.assign(P, 0);
+  out.p_lg.assign(P, 0);
+  // a row-aligned boundary at or after entry k of chunk c
+  auto row_aligned = [&](int c, long long k) {
+    const long long x0 = xbase[c], x1 = xbase[c + 1];
+    while (k < x1 && k > x0 && out.x_l[k] == out.x_l[k - 1]) k++;
+    return k;
+  };
+  par_for((size_t)C, [&](size_t c0, size_t c1) {
+    for (size_t c = c0; c < c1; c++) {
+      const int np = part0[c + 1] - part0[c];
+      if (np == 0) continue;
+      const long long x0 = xbase[c], E = xbase[c + 1] - x0;
+      std::vector<long long> xb(np + 1);
+      for (int q = 0; q <= np; q++) xb[q] = row_aligned((int)c, x0 + E * q / np);
+      xb[np] = xbase[c + 1];
+      for (int q = 0; q < np; q++) {
+        const int p = part0[c] + q;
+        out.p_chunk[p] = (int)c;
+        out.p_l0[p] = q == 0 ? 0 : (xb[q] < xbase[c + 1] ? out.x_l[xb[q]] : out.m[c]);
+      }
+      for (int q = 0; q < np; q++) {
+        const int p = part0[c] + q;
+        out.p_l1[p] = q + 1 < np ? out.p_l0[p + 1] : out.m[c];
+        // lanes per row: the mean row of the part in one round of
+        // kChunkExtBatch gathers per lane
+        const int rows = std::max(1, out.p_l1[p] - out.p_l0[p]);
+        const long long ents = xb[q + 1] - xb[q];
+        int lg = 0;
+        while (lg < 6 && (long long)(kChunkExtBatch << lg) * rows < ents) lg++;
+        out.p_lg[p] = lg;
+      }
+      // the chunk's own run of external entries is then empty
+      int *sp = &out.split[c * (size_t)(kChunkThreads + 1)];
+      for (int t = 0; t <= kChunkThreads; t++) sp[t] = (int)x0;
+    }
+  });
+  // work items: chunks 0..C-1, parts C..C+P-1.  Dependencies: a part waits
+  // for the chunks its chunk reads, a chunk with parts for its parts, any
+  // other chunk for the chunks it reads.  Dequeue order: by depth in the
+  // chunk graph, parts before their chunk, then in sweep order.  And the
+  // critical path estimate.
   std::vector<int> chunk_of(n);
   for (int c = 0; c < C; c++)
     for (int i = b[c]; i < b[c + 1]; i++) chunk_of[i] = c;
@@ -249,7 +315,6 @@ void plan_chunks(const HostSweep &h, const std::vector<int> &slot_of, int n,
   std::vector<double> finish(C, 0.0);
   std::vector<std::vector<int>> deps(C);
   par_for((size_t)C, [&](size_t c0, size_t c1) {
-    std::vector<char> seen;
     for (size_t c = c0; c < c1; c++) {
       std::vector<int> &d = deps[c];
       for (long long e = xbase[c]; e < xbase[c + 1]; e++) d.push_back(chunk_of[out.x_ci[e]]);
@@ -257,30 +322,52 @@ void plan_chunks(const HostSweep &h, const std::vector<int> &slot_of, int n,
       d.erase(std::unique(d.begin(), d.end()), d.end());
     }
   });
-  out.dep_ptr.assign(C + 1, 0);
-  for (int c = 0; c < C; c++) out.dep_ptr[c + 1] = out.dep_ptr[c] + (int)deps[c].size();
-  out.dep.resize(out.dep_ptr[C]);
-  for (int c = 0; c < C; c++) std::copy(deps[c].begin(), deps[c].end(), out.dep.begin() + out.dep_ptr[c]);
+  const int I = C + P;
+  out.dep_ptr.assign(I + 1, 0);
+  for (int c = 0; c < C; c++)
+    out.dep_ptr[c + 1] = part0[c + 1] > part0[c] ? part0[c + 1] - part0[c] : (int)deps[c].size();
+  for (int p = 0; p < P; p++) out.dep_ptr[C + p + 1] = (int)deps[out.p_chunk[p]].size();
+  for (int it = 0; it < I; it++) out.dep_ptr[it + 1] += out.dep_ptr[it];
+  out.dep.resize(out.dep_ptr[I]);
+  for (int c = 0; c < C; c++) {
+    int *d = &out.dep[out.dep_ptr[c]];
+    if (part0[c + 1] > part0[c]) {
+      for (int p = part0[c]; p < part0[c + 1]; p++) *d++ = C + p;
+    } else {
+      for (int q : deps[c]) *d++ = q;
+    }
+  }
+  for (int p = 0; p < P; p++)
+    std::copy(deps[out.p_chunk[p]].begin(), deps[out.p_chunk[p]].end(),
+              out.dep.begin() + out.dep_ptr[C + p]);
   double crit = 0.0;
   for (int t = 0; t < C; t++) {
     const int c = backward ? C - 1 - t : t;
     int dep = 0;
     double start = 0.0;
-    for (int p : deps[c]) {
-      dep = std::max(dep, depth[p] + 1);
-      start = std::max(start, finish[p]);
+    for (int q : deps[c]) {
+      dep = std::max(dep, depth[q] + 1);
+      start = std::max(start, finish[q]);
     }
     depth[c] = dep;
-    const double batches =
-        std::ceil((double)(xbase[c + 1] - xbase[c]) / (kChunkThreads * kChunkExtBatch));
-    finish[c] = start + kChunkFixedUs + rounds_of[c] * kChunkRoundUs + batches * kChunkGatherUs;
+    const long long E = xbase[c + 1] - xbase[c];
+    const int np = part0[c + 1] - part0[c];
+    const double per = np ? (double)E / np + kChunkThreads : (double)E;  // entries of the slowest run
+    const double batches = std::ceil(per / (kChunkThreads * kChunkExtBatch));
+    finish[c] = start + (np ? 2 : 1) * kChunkFixedUs + batches * kChunkGatherUs +
+                rounds_of[c] * kChunkRoundUs;
     crit = std::max(crit, finish[c]);
   }
   out.est_us = crit;
-  out.order.resize(C);
-  for (int c = 0; c < C; c++) out.order[c] = c;
-  std::stable_sort(out.order.begin(), out.order.end(), [&](int p, int q) {
-    if (depth[p] != depth[q]) return depth[p] < depth[q];
-    return backward ? p > q : p < q;
+  out.order.resize(I);
+  for (int it = 0; it < I; it++) out.order[it] = it;
+  auto key_chunk = [&](int it) { return it < C ? it : out.p_chunk[it - C]; };
+  std::stable_sort(out.order.begin(), out.order.end(), [&](int x, int y) {
+    const int cx = key_chunk(x), cy = key_chunk(y);
+    if (depth[cx] != depth[cy]) return depth[cx] < depth[cy];
+    const int kx = x >= C ? 0 : 1, ky = y >= C ? 0 : 1;  // parts before chunks
+    if (kx != ky) return kx < ky;
+    if (cx != cy) return backward ? cx > cy : cx < cy;
+    return x < y;
   });
 }

@@ -29,6 +29,7 @@ struct DevScalars {
   int resid_period; // true-residual replacement period (0 = off)
   int strict;       // 1: grid reductions publish partials with release/acquire fences
   unsigned int ctr[8];  // last-block counters of the reductions (self-resetting)
+  unsigned long long arrive;  // arrivals at the persistent sweeps' residency check
 };
 
 // ctr[CTR_CHUNK_Q] / [CTR_CHUNK_Q + 1]: chunk queue and exit count of the
@@ -268,6 +269,44 @@ __device__ __forceinline__ bool grid_reduce(double (&v)[NV], double *partials, u
     __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   return threadIdx.x == 0;
+}
+
+// Residency check of a persistent grid.  Every workgroup arrives here once,
+// before its first dependency wait, and goes on once the whole grid has
+// arrived: the level-scheduled sweeps can only finish when all of their
+// workgroups are resident together (a wave may wait for a row of any other
+// wave).  A grid that does not fit -- too many workgroups for the CUs, or
+// CUs held by other work -- fails here after a bounded wait: the error flag
+// gets bit 2 and the function returns false in every workgroup (which then
+// skips its tasks but still takes part in the kernel's grid reduction).
+// The counter is never reset: each launch adds exactly gridDim.x arrivals,
+// so the g-th launch of a grid waits for (g + 1) * gridDim.x; the host
+// zeroes it whenever the grid size changes.
+__device__ __forceinline__ bool grid_register(unsigned long long *arrive, int *err) {
+  __shared__ int ok;
+  if (threadIdx.x == 0) {
+    const unsigned long long v =
+        __hip_atomic_fetch_add(arrive, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long target = (v / gridDim.x + 1) * gridDim.x;
+    int good = 1, backoff = 1;
+    const long long t0 = wall_clock64();
+    for (int polls = 1;
+         __hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target;
+         polls++) {
+      for (int i = 0; i < backoff; i++) __builtin_amdgcn_s_sleep(1);
+      if (backoff < 8) backoff <<= 1;
+      if ((polls & 15) == 0 &&
+          (wall_clock64() - t0 > kWaitTicks ||
+           __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+        atomicOr(err, 4);
+        good = 0;
+        break;
+      }
+    }
+    ok = good;
+  }
+  __syncthreads();
+  return ok != 0;
 }
 
 }  // namespace rchol_amd

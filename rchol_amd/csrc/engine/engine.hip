@@ -53,6 +53,16 @@ int fail(int code, const char *fmt, ...) {
   return code;
 }
 
+// The error flag of a sweep as a message: bit 0 a dependency wait hit its
+// bound, bit 1 a schedule held a task kind the kernel does not run, bit 2
+// the persistent grid's workgroups were not all resident at once.
+int stall(int err) {
+  return fail(RCHOL_AMD_ESTALL, "triangular sweep stalled (err=%d:%s%s%s)", err,
+              (err & 1) ? " a dependency wait exceeded its bound" : "",
+              (err & 2) ? " a task kind outside its schedule" : "",
+              (err & 4) ? " not every workgroup of the persistent sweep was resident" : "");
+}
+
 #define HIP_TRY(expr)                                                                    \
   do {                                                                                   \
     hipError_t e_ = (expr);                                                              \
@@ -127,21 +137,27 @@ struct Sweep {
   DevBuf<float> cv32, diag32;
   // chunked form (k_sptrsv_chunk), built when the factor's chains are deep
   struct Chunks {
-    int nchunks = 0;
+    int nchunks = 0, nparts = 0;
     long long rounds = 0;
     double est_us = 0.0;
     DevBuf<int> order, q0, m, r0, r1, split, q_row, x_l, x_ci, dep_ptr, dep;
+    DevBuf<int> c_part0, p_chunk, p_l0, p_l1, p_lg, x_rp;  // external parts
+    DevBuf<double> ext64;  // the parts' external sums (sweep precision)
+    DevBuf<float> ext32;
     DevBuf<int> done;  // per chunk: published (reset by each launch's last workgroup)
     DevBuf<unsigned> r_w;
     DevBuf<double> q_diag64, x_cv64, r_v64;
     DevBuf<float> q_diag32, x_cv32, r_v32;
     void reset() {
       nchunks = 0;
+      nparts = 0;
       rounds = 0;
       est_us = 0.0;
       for (DevBuf<int> *b : {&order, &q0, &m, &r0, &r1, &split, &q_row, &x_l, &x_ci, &dep_ptr, &dep,
-                             &done})
+                             &done, &c_part0, &p_chunk, &p_l0, &p_l1, &p_lg, &x_rp})
         b->reset();
+      ext64.reset();
+      ext32.reset();
       r_w.reset();
       q_diag64.reset();
       x_cv64.reset();
@@ -647,15 +663,19 @@ void launch_one_sweep(rchol_amd_ctx *c, const Sweep &sw, const VT *cv, const VT 
   if (c->use_chunks && sw.ch.nchunks > 0) {
     const Sweep::Chunks &h = sw.ch;
     const VT *qd, *xcv, *rv;
+    XT *ext;
     if constexpr (sizeof(VT) == 8) {
       qd = h.q_diag64.p; xcv = h.x_cv64.p; rv = h.r_v64.p;
     } else {
       qd = h.q_diag32.p; xcv = h.x_cv32.p; rv = h.r_v32.p;
     }
+    if constexpr (sizeof(XT) == 8) ext = (XT *)h.ext64.p; else ext = (XT *)h.ext32.p;
     hipLaunchKernelGGL((k_sptrsv_chunk<VT, XT, BT, FUSE>), dim3(c->chunk_grid), dim3(kChunkThreads),
-                       c->chunk_lds, c->stream, h.nchunks, h.order.p, h.q0.p, h.m.p, h.r0.p,
-                       h.r1.p, h.split.p, h.dep_ptr.p, h.dep.p, h.done.p, h.q_row.p, qd, h.x_l.p,
-                       h.x_ci.p, xcv, h.r_w.p, rv, rhs, x, aux, dotv, c->partials.p, sc, stop);
+                       c->chunk_lds, c->stream, h.nchunks, h.nchunks + h.nparts, h.order.p, h.q0.p,
+                       h.m.p, h.r0.p, h.r1.p, h.split.p, h.c_part0.p, h.p_chunk.p, h.p_l0.p,
+                       h.p_l1.p, h.p_lg.p, h.x_rp.p, h.dep_ptr.p, h.dep.p, h.done.p, ext, h.q_row.p, qd,
+                       h.x_l.p, h.x_ci.p, xcv, h.r_w.p, rv, rhs, x, aux, dotv, c->partials.p, sc,
+                       stop);
     return;
   }
   const bool head = sw.head_end > 0, tail = sw.tail_begin < sw.nlevels;
@@ -805,6 +825,8 @@ int rchol_amd_ctx::size_trsv_grid() {
     max_tasks = std::max(max_tasks, tc[k].s0.size());
   }
   trsv_grid = (int)std::max<size_t>(1, std::min<size_t>(cap_blocks, (max_tasks + 3) / 4));
+  // the residency check counts arrivals in whole grids: restart the count
+  HIP_TRY(hipMemset(&sc->arrive, 0, sizeof(sc->arrive)));
   for (int k = 0; k < 2; k++) HIP_TRY(upload_schedule(*sw[k], h[k], t[k], lmin_lg, tc[k], trsv_grid));
   stats.trsv_waves = trsv_grid * (kBlock / 64);
   stats.fwd_tasks = fwd.mid_ntasks;
@@ -939,7 +961,7 @@ int rchol_amd_ctx::pcg(const double *d_b, double tol, int maxit, double *d_x_out
     if (hsc->done) break;
   }
   if (hsc->err)
-    return fail(RCHOL_AMD_ESTALL, "triangular sweep wait exceeded its bound (err=%d)", hsc->err);
+    return stall(hsc->err);
   // the last completed iteration wrote p_new into p1 when k is odd
   launch_spmv_res(this, x.p, d_b, tmp64a.p);
   HIP_TRY(hipGetLastError());
@@ -983,7 +1005,7 @@ int rchol_amd_ctx::precond(const double *d_r, double *d_z) {
   HIP_TRY(hipMemcpyAsync(&hsc->err, &sc->err, sizeof(int), hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
   if (hsc->err)
-    return fail(RCHOL_AMD_ESTALL, "triangular sweep wait exceeded its bound (err=%d)", hsc->err);
+    return stall(hsc->err);
   return RCHOL_AMD_OK;
 }
 
@@ -1062,6 +1084,7 @@ int rchol_amd_ctx::ensure_batch_schedule(int K, int per_cu) {
     max_tasks = std::max(max_tasks, tc[k].s0.size());
   }
   batch.grid = (int)std::max<size_t>(1, std::min<size_t>(cap_blocks, (max_tasks + 3) / 4));
+  HIP_TRY(hipMemset(&batch.bs->arrive, 0, sizeof(batch.bs->arrive)));
   for (int k = 0; k < 2; k++) {
     std::vector<int> wp, ws0, winfo;
     assign_waves(tc[k], batch.grid, wp, ws0, winfo);
@@ -1225,7 +1248,7 @@ int rchol_amd_ctx::pcg_batch(const double *d_b, int ncols, double tol, int maxit
     if (batch.hbs->err || batch.hbs->all_done) break;
   }
   if (batch.hbs->err)
-    return fail(RCHOL_AMD_ESTALL, "triangular sweep wait exceeded its bound (err=%d)", batch.hbs->err);
+    return stall(batch.hbs->err);
   const int rows_per_block = (kBlock / 64) * (64 / K);
   int sg = (int)std::max<size_t>(
       1, std::min<size_t>((n + rows_per_block - 1) / rows_per_block, (size_t)ncu * kSpmvMrBlocksPerCu));
@@ -1278,7 +1301,7 @@ int rchol_amd_ctx::precond_batch(const double *d_r, double *d_z) {
   HIP_TRY(hipMemcpyAsync(&batch.hbs->err, &batch.bs->err, sizeof(int), hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
   if (batch.hbs->err)
-    return fail(RCHOL_AMD_ESTALL, "triangular sweep wait exceeded its bound (err=%d)", batch.hbs->err);
+    return stall(batch.hbs->err);
   return RCHOL_AMD_OK;
 }
 
@@ -1455,7 +1478,7 @@ int rchol_amd_ctx::pcg_segments(const double *d_b, double tol, int maxit, double
     if (hsc->err || segs.hss->all_done) break;
   }
   if (hsc->err)
-    return fail(RCHOL_AMD_ESTALL, "triangular sweep wait exceeded its bound (err=%d)", hsc->err);
+    return stall(hsc->err);
   switch (spmv_lanes) {
     case 0: hipLaunchKernelGGL(k_spmv_res_seg_s, dim3(segs.nblocks), dim3(kBlock), 0, stream, lay, arp.p, aci.p, av.p, (const double *)x.p, d_b, tmp64a.p, partials.p, segs.ss); break;
     case 4: hipLaunchKernelGGL((k_spmv_res_seg<4>), dim3(segs.nblocks), dim3(kBlock), 0, stream, lay, arp.p, aci.p, av.p, (const double *)x.p, d_b, tmp64a.p, partials.p, segs.ss); break;
@@ -1840,8 +1863,20 @@ int upload_chunks(Sweep::Chunks &d, const HostChunks &h, int precision) {
   HIP_TRY(d.r_w.upload(h.r_w.data(), h.r_w.size()));
   HIP_TRY(d.dep_ptr.upload(h.dep_ptr.data(), h.dep_ptr.size()));
   HIP_TRY(d.dep.upload(h.dep.data(), h.dep.size()));
-  HIP_TRY(d.done.alloc(h.nchunks));
-  HIP_TRY(hipMemset(d.done.p, 0, std::max(h.nchunks, 1) * sizeof(int)));
+  const int items = h.nchunks + h.nparts;
+  HIP_TRY(d.done.alloc(items));
+  HIP_TRY(hipMemset(d.done.p, 0, std::max(items, 1) * sizeof(int)));
+  HIP_TRY(d.c_part0.upload(h.c_part0.data(), h.c_part0.size()));
+  HIP_TRY(d.p_chunk.upload(h.p_chunk.data(), h.p_chunk.size()));
+  HIP_TRY(d.p_l0.upload(h.p_l0.data(), h.p_l0.size()));
+  HIP_TRY(d.p_l1.upload(h.p_l1.data(), h.p_l1.size()));
+  HIP_TRY(d.p_lg.upload(h.p_lg.data(), h.p_lg.size()));
+  HIP_TRY(d.x_rp.upload(h.x_rp.data(), h.x_rp.size()));
+  if (precision == RCHOL_AMD_FP32)
+    HIP_TRY(d.ext32.alloc(h.q_row.size()));
+  else
+    HIP_TRY(d.ext64.alloc(h.q_row.size()));
+  d.nparts = h.nparts;
   if (precision == RCHOL_AMD_FP32) {
     HIP_TRY(upload_as(d.q_diag32, h.q_diag));
     HIP_TRY(upload_as(d.x_cv32, h.x_cv));
@@ -2161,8 +2196,7 @@ int rchol_amd_precond_device_noalloc(rchol_amd_ctx *c, const double *d_r, double
   HIP_TRY(hipMemcpyAsync(&c->hsc->err, &ctl->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (c->hsc->err)
-    return fail(RCHOL_AMD_ESTALL, "triangular sweep wait exceeded its bound (err=%d)",
-                c->hsc->err);
+    return stall(c->hsc->err);
   return RCHOL_AMD_OK;
 }
 
@@ -2551,44 +2585,85 @@ namespace {
 // entries starts on a row boundary, a round reads only rows that no later
 // round writes.
 int simulate_chunks(const HostChunks &p, int n, const double *rhs, double *out) {
-  std::vector<char> done(n, 0), listed(n, 0);
-  std::vector<double> yl;
+  const int C = p.nchunks, I = C + p.nparts;
+  std::vector<char> done(n, 0), listed(n, 0), item_done(I, 0);
+  std::vector<double> yl, extsum(n, 0.0);
   std::vector<int> last;
-  for (int k = 0; k < p.nchunks; k++) {
-    const int c = p.order[k];
-    const int q0 = p.q0[c], m = p.m[c];
-    yl.assign(m, 0.0);
-    for (int l = 0; l < m; l++) yl[l] = rhs[p.q_row[q0 + l]];
-    // rows of the chunks this one waits for (cleared again below)
-    auto mark = [&](char v) {
-      for (int d = p.dep_ptr[c]; d < p.dep_ptr[c + 1]; d++) {
-        const int cd = p.dep[d];
-        for (int l = 0; l < p.m[cd]; l++) listed[p.q_row[p.q0[cd] + l]] = v;
-      }
-    };
-    mark(1);
-    const int *sp = &p.split[(size_t)c * (kChunkThreads + 1)];
+  // rows of the chunks an item may read: those of the chunks it waits for,
+  // through its parts for a chunk with parts
+  auto mark = [&](int it, char v) {
+    for (int d = p.dep_ptr[it]; d < p.dep_ptr[it + 1]; d++) {
+      const int cd = p.dep[d];
+      if (cd >= C) continue;
+      for (int l = 0; l < p.m[cd]; l++) listed[p.q_row[p.q0[cd] + l]] = v;
+    }
+  };
+  // thread runs of external entries, summed per row in entry order
+  auto runs = [&](const int *sp, int it, auto &&sink) -> int {
     for (int t = 0; t < kChunkThreads; t++) {
-      if (sp[t] > sp[t + 1]) return fail(RCHOL_AMD_EINVAL, "chunk %d: split not monotone", c);
+      if (sp[t] > sp[t + 1]) return fail(RCHOL_AMD_EINVAL, "item %d: split not monotone", it);
       if (t > 0 && sp[t] > sp[0] && sp[t] < sp[kChunkThreads] && p.x_l[sp[t]] == p.x_l[sp[t] - 1])
-        return fail(RCHOL_AMD_EINVAL, "chunk %d: thread %d starts inside a row", c, t);
+        return fail(RCHOL_AMD_EINVAL, "item %d: thread %d starts inside a row", it, t);
       int cur = -1;
       double acc = 0.0;
       for (int e = sp[t]; e < sp[t + 1]; e++) {
         const int j = p.x_ci[e];
-        if (!done[j]) return fail(RCHOL_AMD_EINVAL, "chunk %d reads row %d before it is published", c, j);
+        if (!done[j]) return fail(RCHOL_AMD_EINVAL, "item %d reads row %d before it is published", it, j);
         if (!listed[j])
-          return fail(RCHOL_AMD_EINVAL, "chunk %d reads row %d of a chunk it does not wait for", c, j);
+          return fail(RCHOL_AMD_EINVAL, "item %d reads row %d of a chunk it does not wait for", it, j);
         if (p.x_l[e] != cur) {
-          if (cur >= 0) yl[cur] -= acc;
+          if (cur >= 0) sink(cur, acc);
           cur = p.x_l[e];
           acc = 0.0;
         }
         acc += p.x_cv[e] * out[j];
       }
-      if (cur >= 0) yl[cur] -= acc;
+      if (cur >= 0) sink(cur, acc);
     }
-    mark(0);
+    return RCHOL_AMD_OK;
+  };
+  for (int k = 0; k < I; k++) {
+    const int it = p.order[k];
+    for (int d = p.dep_ptr[it]; d < p.dep_ptr[it + 1]; d++)
+      if (!item_done[p.dep[d]])
+        return fail(RCHOL_AMD_EINVAL, "item %d dequeued before item %d it waits for", it, p.dep[d]);
+    if (it >= C) {  // a part: external sums of rows [l0, l1) of its chunk
+      const int q = it - C, c = p.p_chunk[q], q0 = p.q0[c];
+      if (p.p_lg[q] < 0 || p.p_lg[q] > 6) return fail(RCHOL_AMD_EINVAL, "part %d: lane width", q);
+      mark(it, 1);
+      for (int l = p.p_l0[q]; l < p.p_l1[q]; l++) {
+        double acc = 0.0;
+        for (int e = p.x_rp[q0 + l]; e < p.x_rp[q0 + l + 1]; e++) {
+          const int j = p.x_ci[e];
+          if (p.x_l[e] != l) return fail(RCHOL_AMD_EINVAL, "part %d: entry %d of another row", q, e);
+          if (!done[j] || !listed[j])
+            return fail(RCHOL_AMD_EINVAL, "part %d reads row %d before it is published", q, j);
+          acc += p.x_cv[e] * out[j];
+        }
+        extsum[q0 + l] = acc;
+      }
+      mark(it, 0);
+      item_done[it] = 1;
+      continue;
+    }
+    const int c = it;
+    const int q0 = p.q0[c], m = p.m[c];
+    const bool parted = p.c_part0[c + 1] > p.c_part0[c];
+    yl.assign(m, 0.0);
+    for (int l = 0; l < m; l++) yl[l] = rhs[p.q_row[q0 + l]] - (parted ? extsum[q0 + l] : 0.0);
+    if (parted) {  // the parts cover every row, in order
+      int next = 0;
+      for (int q = p.c_part0[c]; q < p.c_part0[c + 1]; q++) {
+        if (p.p_l0[q] != next) return fail(RCHOL_AMD_EINVAL, "chunk %d: parts leave a gap", c);
+        next = p.p_l1[q];
+      }
+      if (next != m) return fail(RCHOL_AMD_EINVAL, "chunk %d: parts end before its rows", c);
+    }
+    mark(it, 1);
+    int rc = runs(&p.split[(size_t)c * (kChunkThreads + 1)], it,
+                  [&](int l, double a) { yl[l] -= a; });
+    mark(it, 0);
+    if (rc) return rc;
     for (int l = 0; l < m; l++) yl[l] = yl[l] / p.q_diag[q0 + l];
     last.assign(m, -1);
     for (int r = p.r0[c]; r < p.r1[c]; r++)
@@ -2615,6 +2690,7 @@ int simulate_chunks(const HostChunks &p, int n, const double *rhs, double *out) 
       out[row] = yl[l];
       done[row] = 1;
     }
+    item_done[it] = 1;
   }
   for (int i = 0; i < n; i++)
     if (!done[i]) return fail(RCHOL_AMD_EINVAL, "row %d is in no chunk", i);
@@ -2627,7 +2703,7 @@ extern "C" {
 
 int rchol_amd_chunk_apply_host(size_t n, const size_t *colPtr, const size_t *rowIdx,
                                const double *val, int max_rows, const double *r, double *z,
-                               int *nchunks, long long *rounds, double *est_us) {
+                               int *counts, long long *rounds, double *est_us) {
   if (!colPtr || !rowIdx || !val || !r || !z) return fail(RCHOL_AMD_EINVAL, "null argument");
   if (n == 0 || n > kIdxMax) return fail(RCHOL_AMD_ERANGE, "n=%zu outside [1, 2^31)", n);
   if (colPtr[0] != 0) return fail(RCHOL_AMD_EINVAL, "colPtr[0] must be 0");
@@ -2651,7 +2727,11 @@ int rchol_amd_chunk_apply_host(size_t n, const size_t *colPtr, const size_t *row
   rc = simulate_chunks(pf, (int)n, r, y.data());
   if (!rc) rc = simulate_chunks(pb, (int)n, y.data(), z);
   if (rc) return rc;
-  if (nchunks) *nchunks = pf.nchunks;
+  if (counts) {
+    counts[0] = pf.nchunks;
+    counts[1] = pf.nparts;
+    counts[2] = pb.nparts;
+  }
   if (rounds) *rounds = pf.rounds + pb.rounds;
   if (est_us) {
     est_us[0] = pf.est_us;

@@ -23,7 +23,8 @@ namespace rchol_amd {
 // launched wave is resident (the grid is sized from the occupancy query),
 // each wave walks its list in level order and a task only depends on rows
 // of lower levels, so among the unfinished tasks of the lowest level, the
-// one its wave is on has all inputs.
+// one its wave is on has all inputs.  That all workgroups are resident is
+// checked at entry (grid_register), not assumed.
 //
 //   x[row] = (rhs[row] - sum_e val[e] * x[col[e]]) / diag[slot]
 //
@@ -254,11 +255,12 @@ __global__ __launch_bounds__(256) void k_sptrsv(
   double dacc = 0.0;
   __shared__ XT sbuf[4][64 * kStream];  // one stream-task product buffer per wave
   XT *buf = sbuf[(threadIdx.x >> 6) & 3];
+  const bool resident = grid_register(&sc->arrive, err);
   const int k1 = wave_ptr[wave + 1];
   // Software pipeline over the wave's list: the descriptor and per-slot
   // metadata of task k+1 are loaded before task k runs, so a task starts
   // with its rows, entry ranges and pivots already in registers.
-  int k = wave_ptr[wave];
+  int k = resident ? wave_ptr[wave] : k1;
   int n_info = 0, n_s0 = 0, n_row = 0, n_e0 = 0, n_e1 = 0;
   VT n_dg = 1;
   auto fetch = [&](int kk) {

@@ -12,12 +12,13 @@
 // in sweep order ("external").  The host orders each chunk's rows by their
 // dependency level inside the chunk (local index l), and a workgroup runs a
 // chunk in three phases:
-//   0. one thread waits for the done flags of the chunks this one reads (a
-//      host-made list), so that waiting costs one poller per workgroup, not
-//      a poll per value and thread;
+//   0. one thread takes the ticket and waits for the done flags of the work
+//      items this one reads (a host-made list), so that waiting costs one
+//      poller per workgroup, not a poll per value and thread;
 //   1. all threads: yl[l] = (rhs[row] - sum_ext val * x[col]) / diag, the
-//      external values read from the output vector (a value still holding
-//      the sentinel is polled for, as a safety net);
+//      external values read from the output vector with agent-scope loads,
+//      kChunkExtBatch in flight per thread (a value still holding the
+//      sentinel is polled for, as a safety net);
 //   2. one wave: the internal entries, pre-divided by their row's diagonal,
 //      in rounds of 64: lane k of a round applies yl[t] -= v * yl[s] with an
 //      LDS atomic.  The host packs a round only with entries whose source
@@ -27,8 +28,12 @@
 //      prefetched kChunkPrefetch ahead into registers;
 //   3. all threads: x[row] = yl[l] (agent-scope stores, each thread fenced)
 //      and the fused epilogues; then the chunk's done flag (release).
-// Chunks come from a queue in an order where every chunk follows the chunks
-// it reads, so a workgroup only ever waits for chunks already taken by
+// A chunk with many external entries hands their sums to "parts": work
+// items of their own, run by other workgroups in parallel, each writing the
+// sums of a row-aligned range of the chunk's rows into extsum; the chunk
+// waits for its parts instead of the chunks they read, and starts from those
+// sums.  Work items come from a queue in an order where every item follows
+// the items it reads, so a workgroup only ever waits for chunks already taken by
 // running workgroups: no co-residency assumption, any grid size completes.
 // The queue counter and the done flags are reset by the last workgroup to
 // leave, so the kernel replays from a captured graph.
@@ -100,12 +105,98 @@ __device__ __forceinline__ void chunk_rounds(XT *yl, const unsigned *__restrict_
   }
 }
 
+// One thread's row-aligned run [e0, e1) of external entries: the sum of
+// val * x[col] of each row, in entry order, handed to sink(local row, sum).
+// kChunkExtBatch gathers in flight; agent-scope loads (a plain load can hit
+// a line this XCD's L2 kept from before the producer published it).
+template <typename VT, typename XT, typename Sink>
+__device__ __forceinline__ void ext_run(int e0, int e1, const int *__restrict__ x_l,
+                                        const int *__restrict__ x_ci,
+                                        const VT *__restrict__ x_cv, const XT *x, int *err,
+                                        Sink &&sink) {
+  int cur = -1;
+  XT acc = 0;
+  for (int e = e0; e < e1; e += kChunkExtBatch) {
+    int lr[kChunkExtBatch], j[kChunkExtBatch];
+    VT v[kChunkExtBatch];
+    XT xv[kChunkExtBatch];
+#pragma unroll
+    for (int u = 0; u < kChunkExtBatch; u++) {
+      const bool ok = e + u < e1;
+      lr[u] = ok ? ld_stream(x_l + e + u) : -1;
+      j[u] = ok ? ld_stream(x_ci + e + u) : 0;
+      v[u] = ok ? ld_stream(x_cv + e + u) : VT(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kChunkExtBatch; u++)
+      xv[u] = lr[u] >= 0 ? Sent<XT>::load_agent(x + j[u]) : XT(0);
+#pragma unroll
+    for (int u = 0; u < kChunkExtBatch; u++)
+      if (lr[u] >= 0 && Sent<XT>::is(xv[u])) xv[u] = chunk_wait_value(x, j[u], err);
+#pragma unroll
+    for (int u = 0; u < kChunkExtBatch; u++) {
+      if (lr[u] < 0) continue;
+      if (lr[u] != cur) {
+        if (cur >= 0) sink(cur, acc);
+        cur = lr[u];
+        acc = 0;
+      }
+      acc += (XT)v[u] * xv[u];
+    }
+  }
+  if (cur >= 0) sink(cur, acc);
+}
+
+// The external sums of local rows [l0, l1) of a chunk into extsum[q0 + l]
+// (zero for a row without entries): row groups of L lanes, kChunkExtBatch
+// gathers per lane and round, a fixed-order group sum.  Called by every
+// thread of the workgroup.
+template <typename VT, typename XT, int L>
+__device__ __forceinline__ void ext_rows(int l0, int l1, int q0, const int *__restrict__ x_rp,
+                                         const int *__restrict__ x_ci,
+                                         const VT *__restrict__ x_cv, const XT *x, int *err,
+                                         XT *extsum) {
+  constexpr int R = 64 / L;  // rows per wave pass
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int grp = lane / L, sub = lane % L;
+  for (int base = l0 + wave * R; base < l1; base += (kChunkThreads / 64) * R) {
+    const int l = base + grp;
+    const bool live = l < l1;
+    const int e0 = live ? x_rp[q0 + l] : 0, e1 = live ? x_rp[q0 + l + 1] : 0;
+    XT acc = 0;
+    for (int c = e0 + sub; c < e1; c += L * kChunkExtBatch) {
+      int j[kChunkExtBatch];
+      VT v[kChunkExtBatch];
+      XT xv[kChunkExtBatch];
+#pragma unroll
+      for (int u = 0; u < kChunkExtBatch; u++) {
+        const int e = c + u * L;
+        j[u] = e < e1 ? ld_stream(x_ci + e) : -1;
+        v[u] = e < e1 ? ld_stream(x_cv + e) : VT(0);
+      }
+#pragma unroll
+      for (int u = 0; u < kChunkExtBatch; u++)
+        xv[u] = j[u] >= 0 ? Sent<XT>::load_agent(x + j[u]) : XT(0);
+#pragma unroll
+      for (int u = 0; u < kChunkExtBatch; u++)
+        if (j[u] >= 0 && Sent<XT>::is(xv[u])) xv[u] = chunk_wait_value(x, j[u], err);
+#pragma unroll
+      for (int u = 0; u < kChunkExtBatch; u++) acc += (XT)v[u] * xv[u];
+    }
+    acc = group_sum<L>(acc);
+    if (live && sub == 0) Sent<XT>::store_agent(extsum + q0 + l, acc);
+  }
+}
+
 template <typename VT, typename XT, typename BT, int FUSE>
 __global__ __launch_bounds__(kChunkThreads) void k_sptrsv_chunk(
-    int nchunks, const int *__restrict__ order, const int *__restrict__ c_q0,
+    int nchunks, int nitems, const int *__restrict__ order, const int *__restrict__ c_q0,
     const int *__restrict__ c_m, const int *__restrict__ c_r0, const int *__restrict__ c_r1,
-    const int *__restrict__ split, const int *__restrict__ dep_ptr, const int *__restrict__ dep,
-    int *done, const int *__restrict__ q_row, const VT *__restrict__ q_diag,
+    const int *__restrict__ split, const int *__restrict__ c_part0,
+    const int *__restrict__ p_chunk, const int *__restrict__ p_l0, const int *__restrict__ p_l1,
+    const int *__restrict__ p_lg, const int *__restrict__ x_rp, const int *__restrict__ dep_ptr,
+    const int *__restrict__ dep, int *done, XT *extsum, const int *__restrict__ q_row,
+    const VT *__restrict__ q_diag,
     const int *__restrict__ x_l, const int *__restrict__ x_ci, const VT *__restrict__ x_cv,
     const unsigned *__restrict__ r_w, const VT *__restrict__ r_v, const BT *__restrict__ rhs,
     XT *x, XT *__restrict__ aux, const double *__restrict__ dotv, double *__restrict__ partials,
@@ -120,93 +211,102 @@ __global__ __launch_bounds__(kChunkThreads) void k_sptrsv_chunk(
   unsigned *qctr = &sc->ctr[CTR_CHUNK_Q];
   const int tid = threadIdx.x;
   double dacc = 0.0;
+  // Thread 0's queue work -- take a ticket, wait until the item's inputs are
+  // published -- sits in one block before a barrier, at the start and at
+  // the end of every item, never across the loop's back edge: with
+  // thread-0 blocks on both sides of the back edge the compiler let the
+  // other lanes of wave 0 loop back to the next barrier ahead of lane 0's
+  // dequeue (waves then met at barriers out of step and re-ran an item).
+  auto take_item = [&]() {
+    const int kk =
+        (int)__hip_atomic_fetch_add(qctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_chunk = kk;
+    if (kk >= nitems) return;
+    const int nx = order[kk];
+    const long long t0 = wall_clock64();
+    for (int d = dep_ptr[nx]; d < dep_ptr[nx + 1]; d++) {
+      int *flag = done + dep[d];
+      int backoff = 1;
+      for (int polls = 1;
+           !__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); polls++) {
+        for (int i = 0; i < backoff; i++) __builtin_amdgcn_s_sleep(1);
+        if (backoff < 16) backoff <<= 1;
+        if ((polls & 15) == 0 &&
+            (wall_clock64() - t0 > kWaitTicks ||
+             __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+          atomicOr(err, 1);
+          break;
+        }
+      }
+    }
+  };
+  if (tid == 0) take_item();
+  __syncthreads();
   for (;;) {
-    if (tid == 0)
-      s_chunk = (int)__hip_atomic_fetch_add(qctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const int k = s_chunk;
-    if (k >= nchunks) break;
-    const int c = order[k];
-    const int q0 = c_q0[c], m = c_m[c];
-    // phase 0: the chunks read here are published
-    if (tid == 0) {
-      const long long t0 = wall_clock64();
-      for (int d = dep_ptr[c]; d < dep_ptr[c + 1]; d++) {
-        int *flag = done + dep[d];
-        int backoff = 1;
-        for (int polls = 1;
-             !__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); polls++) {
-          for (int i = 0; i < backoff; i++) __builtin_amdgcn_s_sleep(1);
-          if (backoff < 16) backoff <<= 1;
-          if ((polls & 15) == 0 &&
-              (wall_clock64() - t0 > kWaitTicks ||
-               __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-            atomicOr(err, 1);
-            break;
-          }
-        }
-      }
-    }
-    __syncthreads();
+    // wave-uniform in registers the compiler can see as uniform: the loop
+    // exit and the part/chunk branch are scalar branches
+    const int k = __builtin_amdgcn_readfirstlane(s_chunk);
+    if (k >= nitems) break;
+    const int it = __builtin_amdgcn_readfirstlane(order[k]);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    // phase 1a: right-hand sides (the chunk's rows are one index range)
-    for (int l = tid; l < m; l += kChunkThreads) {
-      const int row = ld_stream(q_row + q0 + l);
-      yl[l] = (XT)ld_stream(rhs + row);
-      if (FUSE == 1) aux[row] = Sent<XT>::value();
-    }
-    __syncthreads();
-    // phase 1b: external entries, a row-aligned run per thread, in order
-    {
-      const int e1 = split[(size_t)c * (kChunkThreads + 1) + tid + 1];
-      int cur = -1;
-      XT acc = 0;
-      for (int e = split[(size_t)c * (kChunkThreads + 1) + tid]; e < e1; e += kChunkExtBatch) {
-        int lr[kChunkExtBatch], j[kChunkExtBatch];
-        VT v[kChunkExtBatch];
-        XT xv[kChunkExtBatch];
-#pragma unroll
-        for (int u = 0; u < kChunkExtBatch; u++) {
-          const bool ok = e + u < e1;
-          lr[u] = ok ? ld_stream(x_l + e + u) : -1;
-          j[u] = ok ? ld_stream(x_ci + e + u) : 0;
-          v[u] = ok ? ld_stream(x_cv + e + u) : VT(0);
-        }
-#pragma unroll
-        for (int u = 0; u < kChunkExtBatch; u++) xv[u] = lr[u] >= 0 ? x[j[u]] : XT(0);
-#pragma unroll
-        for (int u = 0; u < kChunkExtBatch; u++)
-          if (lr[u] >= 0 && Sent<XT>::is(xv[u])) xv[u] = chunk_wait_value(x, j[u], err);
-#pragma unroll
-        for (int u = 0; u < kChunkExtBatch; u++) {
-          if (lr[u] < 0) continue;
-          if (lr[u] != cur) {
-            if (cur >= 0) yl[cur] -= acc;
-            cur = lr[u];
-            acc = 0;
-          }
-          acc += (XT)v[u] * xv[u];
-        }
+    if (it >= nchunks) {
+      // an external part: the sums of rows [l0, l1) of its chunk into extsum
+      const int p = it - nchunks, c = p_chunk[p], q0 = c_q0[c];
+      const int l0 = p_l0[p], l1 = p_l1[p];
+      switch (__builtin_amdgcn_readfirstlane(p_lg[p])) {
+        case 0: ext_rows<VT, XT, 1>(l0, l1, q0, x_rp, x_ci, x_cv, x, err, extsum); break;
+        case 1: ext_rows<VT, XT, 2>(l0, l1, q0, x_rp, x_ci, x_cv, x, err, extsum); break;
+        case 2: ext_rows<VT, XT, 4>(l0, l1, q0, x_rp, x_ci, x_cv, x, err, extsum); break;
+        case 3: ext_rows<VT, XT, 8>(l0, l1, q0, x_rp, x_ci, x_cv, x, err, extsum); break;
+        case 4: ext_rows<VT, XT, 16>(l0, l1, q0, x_rp, x_ci, x_cv, x, err, extsum); break;
+        case 5: ext_rows<VT, XT, 32>(l0, l1, q0, x_rp, x_ci, x_cv, x, err, extsum); break;
+        default: ext_rows<VT, XT, 64>(l0, l1, q0, x_rp, x_ci, x_cv, x, err, extsum); break;
       }
-      if (cur >= 0) yl[cur] -= acc;
+    } else {
+      const int c = it;
+      const int q0 = c_q0[c], m = c_m[c];
+      const bool parted = c_part0[c + 1] > c_part0[c];
+      // phase 1a: right-hand sides (the chunk's rows are one index range),
+      // less the external sums of the chunk's parts
+      for (int l = tid; l < m; l += kChunkThreads) {
+        const int row = ld_stream(q_row + q0 + l);
+        XT b = (XT)ld_stream(rhs + row);
+        if (parted) b -= Sent<XT>::load_agent(extsum + q0 + l);
+        yl[l] = b;
+        if (FUSE == 1) aux[row] = Sent<XT>::value();
+      }
+      __syncthreads();
+      // phase 1b: external entries, a row-aligned run per thread, in order
+      // (empty for a chunk with parts)
+      ext_run<VT, XT>(split[(size_t)c * (kChunkThreads + 1) + tid],
+                      split[(size_t)c * (kChunkThreads + 1) + tid + 1], x_l, x_ci, x_cv, x, err,
+                      [&](int l, XT a) { yl[l] -= a; });
+      __syncthreads();
+      // phase 1c: scale by the pivot (the internal entries are pre-divided)
+      for (int l = tid; l < m; l += kChunkThreads)
+        yl[l] = yl[l] / (XT)ld_stream(q_diag + q0 + l);
+      __syncthreads();
+      // phase 2: the chain, one wave
+      if (tid < 64) chunk_rounds<VT, XT>(yl, r_w, r_v, c_r0[c], c_r1[c]);
+      __syncthreads();
+      // phase 3: publish
+      for (int l = tid; l < m; l += kChunkThreads) {
+        const int row = ld_stream(q_row + q0 + l);
+        const XT val = yl[l];
+        Sent<XT>::store_agent(x + row, val);
+        if (FUSE == 2) dacc += dotv[row] * (double)val;
+      }
     }
-    __syncthreads();
-    // phase 1c: scale by the pivot (the internal entries are pre-divided)
-    for (int l = tid; l < m; l += kChunkThreads) yl[l] = yl[l] / (XT)ld_stream(q_diag + q0 + l);
-    __syncthreads();
-    // phase 2: the chain, one wave
-    if (tid < 64) chunk_rounds<VT, XT>(yl, r_w, r_v, c_r0[c], c_r1[c]);
-    __syncthreads();
-    // phase 3: publish
-    for (int l = tid; l < m; l += kChunkThreads) {
-      const int row = ld_stream(q_row + q0 + l);
-      const XT val = yl[l];
-      Sent<XT>::store_agent(x + row, val);
-      if (FUSE == 2) dacc += dotv[row] * (double)val;
-    }
+    // the item is published once every thread's stores are (both kinds end
+    // here: no path of the body skips a barrier another path takes); then
+    // the next ticket
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
-    if (tid == 0) __hip_atomic_store(done + c, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) {
+      __hip_atomic_store(done + it, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      take_item();
+    }
+    __syncthreads();
   }
   // the last workgroup out resets the queue and the done flags for the next
   // launch (every other workgroup has taken its last ticket by then)
@@ -214,8 +314,8 @@ __global__ __launch_bounds__(kChunkThreads) void k_sptrsv_chunk(
     s_last = (int)__hip_atomic_fetch_add(qctr + 1, 1u, __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT) == (int)gridDim.x - 1;
   __syncthreads();
-  if (s_last) {
-    for (int k = tid; k < nchunks; k += kChunkThreads)
+  if (__builtin_amdgcn_readfirstlane(s_last)) {
+    for (int k = tid; k < nitems; k += kChunkThreads)
       __hip_atomic_store(done + k, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (tid == 0) {
       __hip_atomic_store(qctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -25,6 +25,7 @@ struct BatchScalars {
   int strict;  // 1: reductions publish partials with release/acquire fences
   double tol;
   unsigned int ctr[8];
+  unsigned long long arrive;  // arrivals at the sweeps' residency check (grid_register)
 };
 
 // Sum lanes holding the same column inside a wave (lanes c, c+K, c+2K, ...).
@@ -179,8 +180,9 @@ __global__ __launch_bounds__(256) void k_sptrsv_mr(
   const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   int *err = &bs->err;
   double dacc = 0.0;
+  const bool resident = grid_register(&bs->arrive, err);
   const int k1 = wave_ptr[wave + 1];
-  int k = wave_ptr[wave];
+  int k = resident ? wave_ptr[wave] : k1;
   int n_info = 0, n_row = 0, n_e0 = 0, n_e1 = 0;
   VT n_dg = 1;
   auto fetch = [&](int kk) {

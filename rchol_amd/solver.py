@@ -370,9 +370,9 @@ def analyze(G, serial_width=0):
 def chunk_apply_host(G, r, max_rows=65535):
     """The chunked sweeps' plan for G (chunks of at most max_rows rows) run
     on the host in the device kernel's order of operations: z = G^-T G^-1 r.
-    Raises on a plan that breaks its invariants.  Returns (z, forward chunks,
-    rounds of both sweeps, critical-path estimate of [forward, backward] in
-    microseconds)."""
+    Raises on a plan that breaks its invariants.  Returns (z, [chunks,
+    forward parts, backward parts], rounds of both sweeps, critical-path
+    estimate of [forward, backward] in microseconds)."""
     if isinstance(G, tuple):
         cp, ri, gv = G
     else:
@@ -382,14 +382,15 @@ def chunk_apply_host(G, r, max_rows=65535):
     gv, pgv = _f64(gv)
     r, pr = _f64(r)
     z = np.empty_like(r)
-    nch = ctypes.c_int()
+    cnt = np.zeros(3, dtype=np.int32)
     nrd = ctypes.c_longlong()
     est = np.zeros(2)
     lib = _lib.engine()
     _lib.check(lib.rchol_amd_chunk_apply_host(cp.size - 1, pcp, pri, pgv, int(max_rows), pr,
-                                              z.ctypes.data_as(_lib.f64p), ctypes.byref(nch),
-                                              ctypes.byref(nrd), est.ctypes.data_as(_lib.f64p)))
-    return z, nch.value, nrd.value, est
+                                              z.ctypes.data_as(_lib.f64p),
+                                              cnt.ctypes.data_as(_lib.i32p), ctypes.byref(nrd),
+                                              est.ctypes.data_as(_lib.f64p)))
+    return z, cnt.tolist(), nrd.value, est
 
 
 def pcg(A, b, epsilon, maxitr, G, Gt=None):

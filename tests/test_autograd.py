@@ -219,3 +219,34 @@ def test_gpu_two_pending_solves_keep_their_own_factors():
         g_ref = -(lam[u] - lam[v]) * (x[u] - x[v]) / (1 + np.exp(-th))
         g = tt.grad.cpu().numpy()
         assert np.linalg.norm(g - g_ref) <= 1e-8 * np.linalg.norm(g_ref)
+
+
+@pytest.mark.gpu
+def test_gpu_mixed_gradient_matches_fp64_adjoint_at_full_size():
+    """The benchmark's 100^3 instance: the mixed-precision forward + adjoint
+    gradient against the FP64 adjoint (PCG to 1e-13), within 1e-6."""
+    torch = pytest.importorskip("torch")
+    import rchol_amd
+    from rchol_amd import autograd as AG
+    if rchol_amd.device_count() < 1:
+        pytest.skip("no GPU")
+    assert torch.cuda.is_available(), "torch sees no GPU next to the engine"
+    print("case autograd adjoint 100^3", flush=True)
+    side = 100
+    n = side ** 3
+    dev = torch.device("cuda", 0)
+    b0 = np.random.default_rng(5).uniform(-1, 1, n)
+    xs = np.random.default_rng(2).uniform(-1, 1, n)
+    grads = {}
+    for prec, tol in ((rchol_amd.FP32, 1e-10), (rchol_amd.FP64, 1e-13)):
+        prob = AG.GridLaplacian(side, side, side, shift=1e-2, leaves=16, precision=prec,
+                                tol=tol)
+        theta = torch.tensor(np.random.default_rng(1).standard_normal(prob.m), device=dev,
+                             requires_grad=True)
+        x = AG.solve(torch.nn.functional.softplus(theta), torch.tensor(b0, device=dev), prob)
+        ((x - torch.tensor(xs, device=dev)) ** 2).sum().backward()
+        grads[prec] = theta.grad.cpu().numpy()
+    g32, g64 = grads[rchol_amd.FP32], grads[rchol_amd.FP64]
+    err = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
+    print(f"100^3 gradient: mixed vs FP64 adjoint {err:.1e}", flush=True)
+    assert err <= 1e-6

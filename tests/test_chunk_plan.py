@@ -26,10 +26,10 @@ def rel(a, b):
 def test_plan_matches_oracle_on_fixtures(name, rows):
     d = np.load(os.path.join(GOLD, name + ".npz"))
     Gs = (d["g_colptr"], d["g_rowidx"], d["g_val"])
-    z, nch, nrd, est = chunk_apply_host(Gs, d["b"], rows)
+    z, cnt, nrd, est = chunk_apply_host(Gs, d["b"], rows)
     assert rel(z, d["z"]) <= 1e-13
     n = d["g_colptr"].size - 1
-    assert 1 <= nch <= n
+    assert 1 <= cnt[0] <= n
 
 
 @pytest.mark.parametrize("case", ["natural16", "static_nd20", "mis_nd20", "varcoef_static12"])
@@ -47,8 +47,21 @@ def test_plan_matches_oracle_on_deep_factors(case, rows):
         perm, blocks = F.nd_grid(20, 20, 20, 4)
         Gs, _, _ = F.factor(A, perm, blocks, case.split("_")[0], 5)
     r = np.random.default_rng(1).uniform(-1, 1, A.shape[0])
-    z, nch, nrd, est = chunk_apply_host(Gs, r, rows)
+    z, cnt, nrd, est = chunk_apply_host(Gs, r, rows)
     assert rel(z, O.precond(Gs, r)) <= 1e-12
     if rows == 256 and case == "static_nd20":
         # eight leaves at least: the cuts found the nested-dissection blocks
-        assert nch >= 8
+        assert cnt[0] >= 8
+
+
+@pytest.mark.parametrize("rows", [4096, 65535])
+def test_plan_with_external_parts_matches_oracle(rows):
+    """Separators of a 40^3 factor in the reference's order read enough
+    external entries that their sums go to parts (separate work items)."""
+    A = F.laplace_3d(40)
+    perm, blocks = F.nd_grid(40, 40, 40, 4)
+    Gs, _, _ = F.factor(A, perm, blocks, "static", 7)
+    r = np.random.default_rng(2).uniform(-1, 1, A.shape[0])
+    z, cnt, nrd, est = chunk_apply_host(Gs, r, rows)
+    assert cnt[1] > 0 and cnt[2] > 0
+    assert rel(z, O.precond(Gs, r)) <= 1e-12
