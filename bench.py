@@ -61,6 +61,9 @@ def parse():
     ap.add_argument("--batch-cols", type=int, default=32,
                     help="right-hand sides of the batch run on the headline factor (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--no-static", action="store_true",
+                    help="skip the FP64 solve on the reference's (static) order")
+    ap.add_argument("--static-steps", type=int, default=2)
     ap.add_argument("--no-ref", action="store_true", help="skip the 1e-14 FP64 reference solve")
     ap.add_argument("--graph-iters", type=int, default=16)
     ap.add_argument("--trsv-lanes", type=int, default=1,
@@ -260,6 +263,49 @@ def run_c3(args):
             cpu["one_core"] = {"value": t1, "unit": "s", "cores": 1, "iters": io1}
             log(f"CPU, one core: {t1:.1f} s")
 
+    # the drop-in path on the factor the reference's own order produces
+    # (natural order inside the nested-dissection blocks): FP64, where the
+    # engine plans chunked sweeps for its deep dependency chains
+    static = None
+    if not args.no_static:
+        s32.close()
+        s64.close()
+        log("static order (the reference's order inside the ND blocks): factorization")
+        A2 = F.varcoef_3d(s, s, s, 1.5, 1e-3, seeds["A"])
+        t0 = time.perf_counter()
+        Gst, pst, _ = F.factor(A2, perm, blocks, "static", seeds["G"])
+        tfs = time.perf_counter() - t0
+        Ast = F.permute(A2, pst)
+        del A2
+        bst = b[pst.astype(np.int64)]
+        sst = rchol_amd.Solver(Ast, Gst, rchol_amd.FP64, graph_iters=args.graph_iters)
+        stt = sst.stats()
+        log(f"static: nnz(G)={Gst[1].size}, levels {stt['fwd_levels']}, chunked {stt['chunked']} "
+            f"({stt['chunks'][0]} chunks), setup ms {[round(v) for v in stt['setup_ms']]}")
+        sst.pcg(bst, args.tol, args.maxit)
+        ts = []
+        for _ in range(max(1, args.static_steps)):
+            xs_, rrs, its = sst.pcg(bst, args.tol, args.maxit)
+            ts.append(sst.stats()["last_solve_ms"] / 1e3)
+        sst.set_option(rchol_amd.solver.OPT_PROFILE, 1)
+        sst.pcg(bst, args.tol, args.maxit)
+        stp = sst.stats()
+        sst.close()
+        kit = max(stp["kernel_iters"], 1)
+        static = {"time_s": float(np.mean(ts)), "iters": its, "relres": rrs,
+                  "relres_fp64_verified": float(np.linalg.norm(Ast @ xs_ - bst) /
+                                                np.linalg.norm(bst)),
+                  "nnz_g": int(Gst[1].size), "fill": 2 * Gst[1].size / Ast.nnz,
+                  "levels": [stt["fwd_levels"], stt["bwd_levels"]], "chunked": stt["chunked"],
+                  "chunks": stt["chunks"], "chunk_rounds": stt["chunk_rounds"],
+                  "factor_s": tfs,
+                  "setup_ms": dict(zip(("upload_a", "analysis", "schedule", "copies"),
+                                       stt["setup_ms"])),
+                  "kernel_ms_per_iter": {KERNEL_NAMES[i]: stp["kernel_ms"][i] / kit
+                                         for i in range(4)}}
+        log(f"static: {its} iterations, {static['time_s'] * 1e3:.0f} ms/solve (FP64)")
+        del Gst, Ast
+
     res = {
         "metric": "PCG time-to-1e-10 & HBM GB/s, 256^3 var-coef Poisson; mixed vs FP64 GPU",
         "value": solve32,
@@ -293,6 +339,7 @@ def run_c3(args):
         "error": err,
         "parity": parity,
         "batch": batch,
+        "static_order_fp64": static,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

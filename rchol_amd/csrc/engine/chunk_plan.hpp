@@ -13,6 +13,21 @@
 // before it did not (a separator after its children).  Cuts elsewhere only
 // change the speed, never the result.
 constexpr int kChunkMinRows = 256;
+
+// f(i) for i in [0, n) on the host threads, indices handed out one at a time
+// (chunks differ in size by orders of magnitude).
+template <typename F>
+void par_each(size_t n, F &&f) {
+  const size_t T = std::min<size_t>((size_t)host_threads(), n);
+  std::atomic<size_t> next{0};
+  auto run = [&]() {
+    for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
+  };
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < T; t++) th.emplace_back(run);
+  run();
+  for (auto &x : th) x.join();
+}
 constexpr int kChunkWindow = 64;
 
 std::vector<int> chunk_bounds(const HostSweep &hf, const std::vector<int> &slot_of, int n,
@@ -65,10 +80,10 @@ struct HostChunks {
   std::vector<int> dep_ptr, dep;  // the work items each work item waits for (CSR)
   std::vector<int> q_row;
   std::vector<double> q_diag;
-  std::vector<int> x_l, x_ci;
-  std::vector<double> x_cv;
-  std::vector<unsigned> r_w;
-  std::vector<double> r_v;
+  std::vector<int, NoInit<int>> x_l, x_ci;
+  std::vector<double, NoInit<double>> x_cv;
+  std::vector<unsigned, NoInit<unsigned>> r_w;
+  std::vector<double, NoInit<double>> r_v;
   long long rounds = 0;
   double est_us = 0.0;  // critical path estimate of one sweep
 };
@@ -123,16 +138,17 @@ void plan_chunks(const HostSweep &h, const std::vector<int> &slot_of, int n,
   out.m.assign(C, 0);
   out.r0.assign(C, 0);
   out.r1.assign(C, 0);
-  std::vector<int> lidx(n), ent_round(h.ci.size()), rounds_of(C, 0), ext_of(C, 0);
+  std::vector<int> lidx(n), rounds_of(C, 0), ext_of(C, 0);
+  std::vector<int, NoInit<int>> ent_round(h.ci.size());
   auto row_entries = [&](int i, int &e0, int &e1) {
     const int s = slot_of[i];
     e0 = h.rp[s];
     e1 = h.rp[s + 1];
   };
   // pass 1: local order, local index, rounds of each internal entry
-  par_for((size_t)C, [&](size_t c0, size_t c1) {
-    std::vector<int> llev, cnt, fin, rows;
-    for (size_t c = c0; c < c1; c++) {
+  par_each((size_t)C, [&](size_t c) {
+    thread_local std::vector<int> llev, cnt, fin, rows;
+    {
       const int a = b[c], z = b[c + 1], m = z - a;
       llev.assign(m, 0);
       int maxlev = 0, ext = 0;
@@ -200,15 +216,24 @@ void plan_chunks(const HostSweep &h, const std::vector<int> &slot_of, int n,
   out.x_cv.resize(xbase[C]);
   // kChunkPrefetch rounds of padding past the last chunk: the ring's reads
   // run that far ahead
-  out.r_w.assign((size_t)(rtot + kChunkPrefetch) * 64, kChunkPad);
-  out.r_v.assign((size_t)(rtot + kChunkPrefetch) * 64, 0.0);
+  out.r_w.resize((size_t)(rtot + kChunkPrefetch) * 64);
+  out.r_v.resize((size_t)(rtot + kChunkPrefetch) * 64);
+  for (size_t e = (size_t)rtot * 64; e < out.r_w.size(); e++) {
+    out.r_w[e] = kChunkPad;
+    out.r_v[e] = 0.0;
+  }
   out.split.assign((size_t)C * (kChunkThreads + 1), 0);
   out.x_rp[n] = (int)xbase[C];
   // pass 2: fill
-  par_for((size_t)C, [&](size_t c0, size_t c1) {
-    std::vector<int> lane, rows;
-    for (size_t c = c0; c < c1; c++) {
+  par_each((size_t)C, [&](size_t c) {
+    thread_local std::vector<int> lane, rows;
+    {
       const int a = b[c], z = b[c + 1], m = z - a;
+      // this chunk's rounds start empty (the arrays are not initialised)
+      std::fill(out.r_w.begin() + (size_t)out.r0[c] * 64, out.r_w.begin() + (size_t)out.r1[c] * 64,
+                kChunkPad);
+      std::fill(out.r_v.begin() + (size_t)out.r0[c] * 64, out.r_v.begin() + (size_t)out.r1[c] * 64,
+                0.0);
       rows.assign(m, 0);
       for (int i = a; i < z; i++) rows[lidx[i]] = i;
       lane.assign(rounds_of[c], 0);
@@ -274,10 +299,10 @@ void plan_chunks(const HostSweep &h, const std::vector<int> &slot_of, int n,
     while (k < x1 && k > x0 && out.x_l[k] == out.x_l[k - 1]) k++;
     return k;
   };
-  par_for((size_t)C, [&](size_t c0, size_t c1) {
-    for (size_t c = c0; c < c1; c++) {
+  par_each((size_t)C, [&](size_t c) {
+    {
       const int np = part0[c + 1] - part0[c];
-      if (np == 0) continue;
+      if (np == 0) return;
       const long long x0 = xbase[c], E = xbase[c + 1] - x0;
       std::vector<long long> xb(np + 1);
       for (int q = 0; q <= np; q++) xb[q] = row_aligned((int)c, x0 + E * q / np);
@@ -314,13 +339,17 @@ void plan_chunks(const HostSweep &h, const std::vector<int> &slot_of, int n,
   std::vector<int> depth(C, 0);
   std::vector<double> finish(C, 0.0);
   std::vector<std::vector<int>> deps(C);
-  par_for((size_t)C, [&](size_t c0, size_t c1) {
-    for (size_t c = c0; c < c1; c++) {
-      std::vector<int> &d = deps[c];
-      for (long long e = xbase[c]; e < xbase[c + 1]; e++) d.push_back(chunk_of[out.x_ci[e]]);
-      std::sort(d.begin(), d.end());
-      d.erase(std::unique(d.begin(), d.end()), d.end());
+  par_each((size_t)C, [&](size_t c) {
+    std::vector<char> seen(C, 0);
+    std::vector<int> &d = deps[c];
+    for (long long e = xbase[c]; e < xbase[c + 1]; e++) {
+      const int q = chunk_of[out.x_ci[e]];
+      if (!seen[q]) {
+        seen[q] = 1;
+        d.push_back(q);
+      }
     }
+    std::sort(d.begin(), d.end());
   });
   const int I = C + P;
   out.dep_ptr.assign(I + 1, 0);

@@ -1833,11 +1833,20 @@ namespace {
 // the chunked sweeps are planned and their estimate compared with the level
 // schedule's (about kLevelUs per level, measured on MI355X).
 constexpr int kChunkMinLevels = 1024;
+// Rows per chunk: at most what the LDS holds and at most kChunkMaxRows (the
+// local row index has 16 bits).  20352 is what 160 KB of LDS holds in FP64;
+// FP32 chunks of that size run faster than LDS-sized ones (256^3 Poisson in
+// the reference's order: 37.6 vs 40.7 ms per forward sweep).
+#ifndef RCHOL_AMD_CHUNK_ROWS
+#define RCHOL_AMD_CHUNK_ROWS 20352
+#endif
+constexpr size_t kChunkMaxRows = RCHOL_AMD_CHUNK_ROWS;
+static_assert(kChunkMaxRows <= 65535, "chunk rows index 16-bit local rows");
 constexpr double kChunkMaxMeanWidth = 4096.0;
 constexpr double kLevelUs = 1.3;
 
-template <typename T>
-hipError_t upload_as(DevBuf<T> &d, const std::vector<double> &h) {
+template <typename T, typename Vec>
+hipError_t upload_as(DevBuf<T> &d, const Vec &h) {
   if constexpr (sizeof(T) == sizeof(double)) {
     return d.upload((const T *)h.data(), h.size());
   } else {
@@ -1927,7 +1936,7 @@ int setup_chunks(rchol_amd_ctx *c, size_t n, const HostSweep &hf, const HostSwee
   if (!(c->chunk_mode == 1 || (c->chunk_mode == -1 && deep))) return RCHOL_AMD_OK;
   const size_t xt = precision == RCHOL_AMD_FP32 ? sizeof(float) : sizeof(double);
   const size_t budget = c->lds_max > 2048 ? (size_t)c->lds_max - 1024 : 0;
-  const int S = (int)std::min<size_t>(65535, budget / xt - 1);
+  const int S = (int)std::min<size_t>(kChunkMaxRows, budget / xt - 1);
   if (S < kChunkMinRows)
     return c->chunk_mode == 1 ? fail(RCHOL_AMD_EHIP, "%d bytes of LDS per workgroup is too little "
                                      "for the chunked sweeps", c->lds_max)
