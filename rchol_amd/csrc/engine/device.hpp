@@ -271,24 +271,37 @@ __device__ __forceinline__ bool grid_reduce(double (&v)[NV], double *partials, u
   return threadIdx.x == 0;
 }
 
-// Residency check of a persistent grid.  Every workgroup arrives here once,
-// before its first dependency wait, and goes on once the whole grid has
-// arrived: the level-scheduled sweeps can only finish when all of their
-// workgroups are resident together (a wave may wait for a row of any other
-// wave).  A grid that does not fit -- too many workgroups for the CUs, or
-// CUs held by other work -- fails here after a bounded wait: the error flag
-// gets bit 2 and the function returns false in every workgroup (which then
-// skips its tasks but still takes part in the kernel's grid reduction).
+// Residency check of a persistent grid.  The level-scheduled sweeps can
+// only finish when all of their workgroups are resident together (a wave
+// may wait for a row of any other wave), so that is checked, not assumed:
+//  * grid_arrive: every workgroup counts itself in at entry (one atomic, no
+//    wait) and gets the count the whole grid reaches;
+//  * wave_resident: every wave, before its first task that may wait for
+//    another wave, waits (one polling lane, bounded) until that count is
+//    reached.  Tasks that wait for nothing (level 0 of the forward sweep)
+//    run first, while the rest of the grid is still being dispatched.
+// A grid that does not fit -- too many workgroups, or CUs held by other
+// work -- fails the bounded wait: the error flag gets bit 2 and the wave
+// skips its tasks (taking part in the kernel's grid reduction all the same).
 // The counter is never reset: each launch adds exactly gridDim.x arrivals,
 // so the g-th launch of a grid waits for (g + 1) * gridDim.x; the host
 // zeroes it whenever the grid size changes.
-__device__ __forceinline__ bool grid_register(unsigned long long *arrive, int *err) {
-  __shared__ int ok;
+__device__ __forceinline__ unsigned long long grid_arrive(unsigned long long *arrive) {
+  __shared__ unsigned long long target;
   if (threadIdx.x == 0) {
     const unsigned long long v =
         __hip_atomic_fetch_add(arrive, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long target = (v / gridDim.x + 1) * gridDim.x;
-    int good = 1, backoff = 1;
+    target = (v / gridDim.x + 1) * gridDim.x;
+  }
+  __syncthreads();
+  return target;
+}
+
+__device__ __forceinline__ bool wave_resident(const unsigned long long *arrive,
+                                              unsigned long long target, int *err) {
+  int good = 1;
+  if (__lane_id() == 0) {
+    int backoff = 1;
     const long long t0 = wall_clock64();
     for (int polls = 1;
          __hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target;
@@ -303,10 +316,8 @@ __device__ __forceinline__ bool grid_register(unsigned long long *arrive, int *e
         break;
       }
     }
-    ok = good;
   }
-  __syncthreads();
-  return ok != 0;
+  return __shfl(good, 0, 64) != 0;
 }
 
 }  // namespace rchol_amd

@@ -24,7 +24,7 @@ namespace rchol_amd {
 // each wave walks its list in level order and a task only depends on rows
 // of lower levels, so among the unfinished tasks of the lowest level, the
 // one its wave is on has all inputs.  That all workgroups are resident is
-// checked at entry (grid_register), not assumed.
+// checked (grid_arrive / wave_resident), not assumed.
 //
 //   x[row] = (rhs[row] - sum_e val[e] * x[col[e]]) / diag[slot]
 //
@@ -255,12 +255,13 @@ __global__ __launch_bounds__(256) void k_sptrsv(
   double dacc = 0.0;
   __shared__ XT sbuf[4][64 * kStream];  // one stream-task product buffer per wave
   XT *buf = sbuf[(threadIdx.x >> 6) & 3];
-  const bool resident = grid_register(&sc->arrive, err);
+  const unsigned long long arrivals = grid_arrive(&sc->arrive);
+  bool checked = false;
   const int k1 = wave_ptr[wave + 1];
   // Software pipeline over the wave's list: the descriptor and per-slot
   // metadata of task k+1 are loaded before task k runs, so a task starts
   // with its rows, entry ranges and pivots already in registers.
-  int k = resident ? wave_ptr[wave] : k1;
+  int k = wave_ptr[wave];
   int n_info = 0, n_s0 = 0, n_row = 0, n_e0 = 0, n_e1 = 0;
   VT n_dg = 1;
   auto fetch = [&](int kk) {
@@ -281,6 +282,10 @@ __global__ __launch_bounds__(256) void k_sptrsv(
     const int cnt = info & ((1 << kTaskInfoShift) - 1);  // <= 64 slots (flat: 64 kFlatPerLane)
     const int m_row = n_row, m_e0 = n_e0, m_e1 = n_e1;
     const VT m_dg = n_dg;
+    if (!checked && lg != kTaskFlat) {  // the first task that may wait
+      checked = true;
+      if (!wave_resident(&sc->arrive, arrivals, err)) break;
+    }
     if (k + 1 < k1) fetch(k + 1);
 #define RCHOL_SWEEP_TASK(LL)                                                                  \
   sweep_task<VT, XT, BT, LL, FUSE>(cnt, m_row, m_e0, m_e1, m_dg, ci, cv, rhs, x, aux, dotv, err, \

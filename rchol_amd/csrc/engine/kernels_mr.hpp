@@ -180,9 +180,9 @@ __global__ __launch_bounds__(256) void k_sptrsv_mr(
   const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   int *err = &bs->err;
   double dacc = 0.0;
-  const bool resident = grid_register(&bs->arrive, err);
+  const unsigned long long arrivals = grid_arrive(&bs->arrive);
   const int k1 = wave_ptr[wave + 1];
-  int k = resident ? wave_ptr[wave] : k1;
+  int k = wave_resident(&bs->arrive, arrivals, err) ? wave_ptr[wave] : k1;
   int n_info = 0, n_row = 0, n_e0 = 0, n_e1 = 0;
   VT n_dg = 1;
   auto fetch = [&](int kk) {
