@@ -70,7 +70,8 @@ def parse():
                     help="minimum lanes per sweep row (tasks widen to their longest row)")
     ap.add_argument("--spmv-lanes", type=int, default=0,
                     help="lanes per SpMV row; 0 = stream form")
-    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4a", "c4b", "c5"],
+    ap.add_argument("--config", default="c3",
+                    choices=["c1", "c2", "c3", "c4a", "c4b", "c5", "factor"],
                     help="c3 (default) is the headline; the others print their own line")
     ap.add_argument("--nrhs", type=int, default=32)
     ap.add_argument("--nsys", type=int, default=48)
@@ -83,6 +84,8 @@ def main():
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is design-only)", file=sys.stderr)
         return 2
+    if args.config == "factor":
+        return run_factor(args)
     import rchol_amd
     if rchol_amd.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
@@ -571,6 +574,55 @@ def run_c4b(args):
     res.update({"sequential_s": t_seq, "speedup_vs_sequential": t_seq / t_batch,
                 "iters_max": int(it.max()), "iters_min": int(it.min()),
                 "max_true_relres": worst, "autotune": tune})
+    _emit(res)
+    return 0
+
+
+def run_factor(args):
+    """Host only: this package's factorization (librchol_factor.so) against the
+    reference's own (c++/rchol_lap, built from its sources into oracle/_ref),
+    on the same matrix, nested dissection and seed.  --side 128 --leaves 64 is
+    the 7-point Laplacian of the 128^3 workload; other sides use the
+    variable-coefficient problem of the headline.  Each side runs on the
+    threads it spawns (one per leaf, as rchol_lap.cpp:186-263) plus this
+    package's host-parallel loops; the reference pins its leaf threads to
+    cores 0..leaves-1 (rchol_lap.cpp:100-114, :190-197).  The static-order
+    factor is checked bit-identical to the reference's."""
+    import resource
+    from rchol_amd import factor as F
+    from oracle import oracle as O
+    s, leaves, seed = args.side, args.leaves, 1
+    depth = int(np.log2(leaves)) + 1
+    if s == 128:
+        A, what = F.laplace_3d(s), "7-point Laplacian"
+    else:
+        A, what = F.varcoef_3d(s, s, s, 1.5, 1e-3, 20261202), "variable-coefficient Poisson"
+    perm, blocks = F.nd_grid(s, s, s, depth)
+    rows = {}
+    for order in ("static", "mis"):
+        t = time.perf_counter()
+        Gs, po, te = F.factor(A, perm, blocks, order, seed)
+        rows[order] = {"wall_s": time.perf_counter() - t, "elimination_s": te,
+                       "nnz_g": int(Gs[0][-1])}
+        if order == "static":
+            G_static = Gs
+        else:
+            del Gs
+        log(f"factor {order}: {rows[order]}")
+    ref = None
+    if O.ref_available() and not args.no_ref:
+        t = time.perf_counter()
+        Gr = O.ref_rchol_nd(A, perm, blocks, seed)
+        ref = {"wall_s": time.perf_counter() - t, "nnz_g": int(Gr[0][-1]),
+               "bit_identical_to_static": all(np.array_equal(a, b) for a, b in zip(Gr, G_static))}
+        log(f"reference: {ref}")
+    res = _base("factor_seconds", rows["static"]["wall_s"], "s", 1, 0, False,
+                {"workload": f"{s}^3 {what}, ND {leaves} leaves, seed {seed}",
+                 "n": int(A.shape[0]), "nnz_a": int(A.nnz), "leaves": leaves}, "f64")
+    res.update({"this_static": rows["static"], "this_mis": rows["mis"], "reference": ref,
+                "speedup_static_vs_reference": ref["wall_s"] / rows["static"]["wall_s"] if ref else None,
+                "max_rss_gb": resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 2**20,
+                "host": _host_info()})
     _emit(res)
     return 0
 

@@ -1,6 +1,6 @@
 // The reference's C++ driver (c++/ex_laplace.cpp, ex_laplace_parallel.cpp)
 // on this engine: 3D Poisson, seeded RHS, rchol factor, PCG through the
-// drop-in pcg(...) constructor.  Usage: ex_laplace -n SIDE [-t LEAVES]
+// drop-in rchol(...) and pcg(...) entry points.  Usage: ex_laplace -n SIDE [-t LEAVES]
 // [-tol TOL] [-maxit M] [-seed S] [-bfile B] [-xfile X]
 // -bfile reads b (SIDE^3 raw FP64, original numbering) instead of drawing it;
 // -xfile writes the solution x (raw FP64, original numbering).
@@ -13,7 +13,7 @@
 #include <vector>
 
 #include "rchol_amd_pcg.hpp"
-#include "rchol_factor.h"
+#include "rchol_amd_rchol.hpp"
 
 // Layout of the reference SparseCSR (c++/sparse.hpp:26-30).
 struct SparseCSR {
@@ -52,51 +52,29 @@ int main(int argc, char **argv) {
     }
     fclose(fb);
   }
-  // nested dissection with `threads` leaves (identity ordering for 1)
-  int depth = 1;
-  while ((1 << (depth - 1)) < threads) depth++;
-  std::vector<size_t> perm(N), blocks((1u << depth) - 1);
-  if (depth == 1) {
-    for (size_t i = 0; i < N; i++) perm[i] = i;
-    blocks[0] = N;
-  } else if (rchol_nd_grid(n, n, n, depth, perm.data(), blocks.data())) {
-    return 1;
-  }
-  rchol_factor_out f;
-  if (rchol_factor(N, A0.rowPtr, A0.colIdx, A0.val, depth == 1 ? nullptr : perm.data(),
-                   blocks.data(), blocks.size(), RCHOL_ORDER_STATIC, seed, &f)) {
-    fprintf(stderr, "%s\n", rchol_factor_last_error());
-    return 1;
-  }
+  // rchol(A, G, P, threads) as ex_laplace_parallel.cpp:29-32, with the
+  // seed and the grid's geometric dissection made explicit
   SparseCSR G;
-  G.N = f.n;
-  G.rowPtr = f.colPtr;
-  G.colIdx = f.rowIdx;
-  G.val = f.val;
-  // A(P,P) and b(P) as ex_laplace_parallel.cpp:38-39
-  std::vector<size_t> inv(N);
-  for (size_t i = 0; i < N; i++) inv[f.perm[i]] = i;
-  std::vector<size_t> rp(N + 1, 0), ci;
-  std::vector<double> av;
-  std::vector<double> bp(N);
-  for (size_t i = 0; i < N; i++) {
-    const size_t o = f.perm[i];
-    std::vector<std::pair<size_t, double>> row;
-    for (size_t k = A0.rowPtr[o]; k < A0.rowPtr[o + 1]; k++)
-      row.push_back({inv[A0.colIdx[k]], A0.val[k]});
-    std::sort(row.begin(), row.end());
-    for (auto &e : row) {
-      ci.push_back(e.first);
-      av.push_back(e.second);
-    }
-    rp[i + 1] = ci.size();
-    bp[i] = b[o];
+  std::vector<size_t> P, part;
+  rchol_amd::FactorOptions opt;
+  opt.threads = threads;
+  opt.use_seed = true;
+  opt.seed = seed;
+  opt.grid[0] = opt.grid[1] = opt.grid[2] = n;
+  try {
+    rchol_amd::factorize(A0, G, P, part, opt);
+  } catch (const std::exception &e) {
+    fprintf(stderr, "%s\n", e.what());
+    return 1;
   }
+  // A(P,P) and b(P) as ex_laplace_parallel.cpp:38-39
   SparseCSR A;
   A.N = N;
-  A.rowPtr = rp.data();
-  A.colIdx = ci.data();
-  A.val = av.data();
+  if (rchol_permute_csr(N, A0.rowPtr, A0.colIdx, A0.val, P.data(), &A.rowPtr, &A.colIdx,
+                        &A.val))
+    return 1;
+  std::vector<double> bp(N);
+  for (size_t i = 0; i < N; i++) bp[i] = b[P[i]];
   printf("Fill-in ratio: %.3g\n", 2.0 * G.rowPtr[N] / A.rowPtr[N]);
   double relres = 0.0;
   int itr = 0;
@@ -112,7 +90,7 @@ int main(int argc, char **argv) {
   if (xfile) {
     // x of A(P,P) back to the original numbering: x_orig[P[i]] = x[i]
     std::vector<double> xo(N);
-    for (size_t i = 0; i < N; i++) xo[f.perm[i]] = x[i];
+    for (size_t i = 0; i < N; i++) xo[P[i]] = x[i];
     FILE *fx = fopen(xfile, "wb");
     if (!fx || fwrite(xo.data(), sizeof(double), N, fx) != N) {
       fprintf(stderr, "cannot write %s\n", xfile);
@@ -120,10 +98,12 @@ int main(int argc, char **argv) {
     }
     fclose(fx);
   }
-  rchol_factor_free(f.colPtr);
-  rchol_factor_free(f.rowIdx);
-  rchol_factor_free(f.val);
-  rchol_factor_free(f.perm);
+  delete[] G.rowPtr;
+  delete[] G.colIdx;
+  delete[] G.val;
+  rchol_factor_free(A.rowPtr);
+  rchol_factor_free(A.colIdx);
+  rchol_factor_free(A.val);
   rchol_factor_free(A0.rowPtr);
   rchol_factor_free(A0.colIdx);
   rchol_factor_free(A0.val);
