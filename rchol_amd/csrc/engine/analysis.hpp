@@ -2,13 +2,34 @@
 // levels of the forward and the backward sweep, rows in level ("slot")
 // order and their off-diagonal entries in that order.  Plain C++ (no HIP):
 // included by engine.hip inside its anonymous namespace, after fail() and
-// kTaskRows.
+// kTaskRows, and after <algorithm>, <atomic>, <chrono>, <memory>, <mutex>,
+// <thread>, <vector> and <sys/mman.h>.
+//
+// Every pass is parallel over host threads and walks the factor in its own
+// (nested-dissection) order, so the level reads and the scattered writes of
+// a thread stay inside the piece of the domain it works on:
+//   1. columns: validation, pivots, entry counts per row bucket, and which
+//      pieces of rows depend on which;
+//   2. the CSR of lower G in natural row order (a transposition through
+//      row buckets, with no atomic operations);
+//   3. levels of both sweeps over pieces of rows, each piece a task that
+//      runs once every piece it reads is done; each row's entries sorted
+//      for its sweep as its level is found;
+//   4. slots (stable counting sort by level).
+// Rows stay in natural order, each sorted for its sweep in step 3; the
+// device gathers them into slot order (materialize_slots() does it on the
+// host for the chunk planner).
+// The result does not depend on the thread count.
 #pragma once
 
-// Host-side schedule of one sweep before upload.
 // Allocator whose resize leaves new elements uninitialised: the analysis
 // writes every element of the entry arrays from several threads, and a
-// zero-fill first would be a serial pass over gigabytes.
+// zero-fill first would be a serial pass over gigabytes.  Allocations of
+// 64 MiB and more are 2 MiB aligned and advised for transparent huge
+// pages, which makes the threads' first touch of them cheaper.
+constexpr size_t kHugeAllocBytes = 64ull << 20;
+constexpr size_t kHugePage = 2ull << 20;
+
 template <typename T>
 struct NoInit : std::allocator<T> {
   template <typename U>
@@ -18,6 +39,20 @@ struct NoInit : std::allocator<T> {
   NoInit() = default;
   template <typename U>
   NoInit(const NoInit<U> &) {}
+  T *allocate(size_t k) {
+    const size_t bytes = k * sizeof(T);
+    if (bytes < kHugeAllocBytes) return std::allocator<T>::allocate(k);
+    void *p = nullptr;
+    if (posix_memalign(&p, kHugePage, bytes)) throw std::bad_alloc();
+    (void)madvise(p, (bytes + kHugePage - 1) & ~(kHugePage - 1), MADV_HUGEPAGE);
+    return (T *)p;
+  }
+  void deallocate(T *p, size_t k) {
+    if (k * sizeof(T) < kHugeAllocBytes)
+      std::allocator<T>::deallocate(p, k);
+    else
+      free(p);
+  }
   template <typename U>
   void construct(U *p) noexcept {
     ::new ((void *)p) U;
@@ -27,14 +62,44 @@ struct NoInit : std::allocator<T> {
     ::new ((void *)p) U(std::forward<Args>(args)...);
   }
 };
+template <typename T, typename U>
+bool operator==(const NoInit<T> &, const NoInit<U> &) {
+  return true;
+}
+template <typename T, typename U>
+bool operator!=(const NoInit<T> &, const NoInit<U> &) {
+  return false;
+}
 
+using IntVec = std::vector<int, NoInit<int>>;
+
+using DblVec = std::vector<double, NoInit<double>>;
+
+// Host-side schedule of one sweep before upload.  The rows' entries are
+// kept in natural row order (row r at [nat_rp[r], nat_rp[r + 1]) of nat_ci /
+// nat_cv, each row already in sweep order, in `store`); the device gathers
+// them into slot order.  ci / cv / diag hold the slot-order copy on the
+// host only after materialize_slots() (the chunk planner reads it).
 struct HostSweep {
-  std::vector<int> task_ptr, slot_row, rp, level_ptr;
-  std::vector<int, NoInit<int>> ci;
-  std::vector<double, NoInit<double>> cv;
-  std::vector<double> diag;
+  std::vector<int> task_ptr, level_ptr;
+  IntVec slot_row, rp;  // slot s reads row slot_row[s]; its entries at [rp[s], rp[s + 1])
+  IntVec nat_rp;
+  std::vector<char, NoInit<char>> store;
+  int *nat_ci = nullptr;
+  double *nat_cv = nullptr;
+  std::shared_ptr<const DblVec> nat_diag;  // pivots, natural order
+  IntVec ci;
+  DblVec cv, diag;
   int nlevels = 0;
 };
+
+// Room for m entries (int index + double value) in one allocation.
+void alloc_entries(HostSweep &h, size_t m) {
+  const size_t ci_bytes = (m * sizeof(int) + 7) & ~(size_t)7;
+  h.store.resize(std::max<size_t>(ci_bytes + m * sizeof(double), 8));
+  h.nat_ci = reinterpret_cast<int *>(h.store.data());
+  h.nat_cv = reinterpret_cast<double *>(h.store.data() + ci_bytes);
+}
 
 // Host threads for the factor analysis: hardware threads, at most
 // OMP_NUM_THREADS when that is set, at most 32.
@@ -54,111 +119,227 @@ double host_ms() {
       .count();
 }
 
-// f(lo, hi) over [0, n) in contiguous chunks, one per thread (chunks of at
-// least `grain` items; the caller's thread takes the first).
+// f(t) for t in [0, T): T-1 new threads, the caller's thread runs t = 0.
 template <typename F>
-void par_for(size_t n, F &&f, size_t grain = 1 << 16) {
-  const size_t T = std::min<size_t>((size_t)host_threads(), std::max<size_t>(1, n / grain));
+void par_run(int T, F &&f) {
   if (T <= 1) {
-    f((size_t)0, n);
+    f(0);
     return;
   }
   std::vector<std::thread> th;
   th.reserve(T - 1);
-  for (size_t t = 1; t < T; t++) th.emplace_back([&f, n, t, T]() { f(n * t / T, n * (t + 1) / T); });
-  f((size_t)0, n / T);
+  for (int t = 1; t < T; t++) th.emplace_back([&f, t]() { f(t); });
+  f(0);
   for (auto &x : th) x.join();
+}
+
+// f(lo, hi) over [0, n) in contiguous chunks, one per thread (chunks of at
+// least `grain` items; the caller's thread takes the first).
+template <typename F>
+void par_for(size_t n, F &&f, size_t grain = 1 << 16) {
+  const int T = (int)std::min<size_t>((size_t)host_threads(), std::max<size_t>(1, n / grain));
+  par_run(T, [&](int t) { f(n * t / T, n * (t + 1) / T); });
+}
+
+// out[0] = 0, out[i + 1] = out[i] + cnt(i) for i in [0, n): per-thread sums
+// of contiguous ranges, then each range's running sum from its offset.
+template <typename C>
+void par_prefix(size_t n, C &&cnt, IntVec &out) {
+  out.resize(n + 1);
+  const int T = (int)std::min<size_t>((size_t)host_threads(), std::max<size_t>(1, n / (1 << 16)));
+  std::vector<long long> part(T + 1, 0);
+  par_run(T, [&](int t) {
+    long long s = 0;
+    for (size_t i = n * t / T; i < n * (t + 1) / T; i++) s += cnt(i);
+    part[t + 1] = s;
+  });
+  for (int t = 0; t < T; t++) part[t + 1] += part[t];
+  out[0] = 0;
+  par_run(T, [&](int t) {
+    long long s = part[t];
+    for (size_t i = n * t / T; i < n * (t + 1) / T; i++) {
+      s += cnt(i);
+      out[i + 1] = (int)s;
+    }
+  });
+}
+
+// Column ranges for T threads with about equal work (entries + 2 per column).
+std::vector<size_t> column_ranges(size_t n, const size_t *colPtr, int T) {
+  std::vector<size_t> b(T + 1, n);
+  b[0] = 0;
+  const double total = (double)colPtr[n] + 2.0 * n;
+  size_t j = 0;
+  for (int t = 1; t < T; t++) {
+    const double goal = total * t / T;
+    size_t lo = j, hi = n;
+    while (lo < hi) {  // first column whose start reaches the goal
+      const size_t mid = (lo + hi) / 2;
+      if ((double)colPtr[mid] + 2.0 * mid < goal)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    b[t] = j = lo;
+  }
+  return b;
+}
+
+// Tasks of a dependency graph on a pool of T threads: task v may run once
+// every task u with an edge u -> v has finished (succ_ptr/succ: the edges
+// out of each task, indeg: edges into it).  Ready tasks wait on a shared
+// stack; piece(t, v) runs task v on thread t.
+template <typename F>
+void run_dag(int V, const std::vector<int> &succ_ptr, const std::vector<int> &succ,
+             std::vector<int> &indeg, int T, F &&piece) {
+  std::mutex mu;
+  std::vector<int> ready;
+  for (int v = 0; v < V; v++)
+    if (indeg[v] == 0) ready.push_back(v);
+  std::atomic<int> left{V};
+  par_run(T, [&](int t) {
+    for (;;) {
+      int v = -1;
+      {
+        std::lock_guard<std::mutex> g(mu);
+        if (!ready.empty()) {
+          v = ready.back();
+          ready.pop_back();
+        }
+      }
+      if (v < 0) {
+        if (left.load(std::memory_order_acquire) == 0) return;
+        std::this_thread::yield();
+        continue;
+      }
+      piece(t, v);
+      for (int k = succ_ptr[v]; k < succ_ptr[v + 1]; k++) {
+        const int w = succ[k];
+        if (__atomic_sub_fetch(&indeg[w], 1, __ATOMIC_ACQ_REL) == 0) {
+          std::lock_guard<std::mutex> g(mu);
+          ready.push_back(w);
+        }
+      }
+      left.fetch_sub(1, std::memory_order_acq_rel);
+    }
+  });
 }
 
 // Rows sorted by level (stable counting sort: inside a level the rows keep
 // their nested-dissection order, so a run of slots is a compact piece of
-// the domain), then level-chunked 64-slot tasks.
-void build_slots(const std::vector<int> &lev, int nlev, HostSweep &hs, std::vector<int> &slot_of) {
+// the domain), then level-chunked 64-slot tasks.  Threads count their row
+// ranges per level when the per-thread counters are no larger than the rows.
+void build_slots(const IntVec &lev, int nlev, HostSweep &hs, IntVec &slot_of) {
   const int n = (int)lev.size();
-  std::vector<int> cnt(nlev + 1, 0);
-  for (int i = 0; i < n; i++) cnt[lev[i] + 1]++;
-  for (int l = 0; l < nlev; l++) cnt[l + 1] += cnt[l];
-  hs.slot_row.assign(n, 0);
-  slot_of.assign(n, 0);
-  std::vector<int> cur(cnt.begin(), cnt.end() - 1);
-  for (int i = 0; i < n; i++) {
-    int s = cur[lev[i]]++;
-    hs.slot_row[s] = i;
-    slot_of[i] = s;
+  int T = (int)std::min<size_t>((size_t)host_threads(), std::max<size_t>(1, (size_t)n / (1 << 16)));
+  if ((long long)T * (nlev + 1) > n) T = 1;
+  std::vector<int> cnt((size_t)T * (nlev + 1), 0);
+  par_run(T, [&](int t) {
+    int *c = &cnt[(size_t)t * (nlev + 1)];
+    for (int i = (int)((long long)n * t / T); i < (int)((long long)n * (t + 1) / T); i++) c[lev[i]]++;
+  });
+  std::vector<int> lp(nlev + 1, 0);
+  {  // level l of thread t starts after levels < l and after threads < t in level l
+    int s = 0;
+    for (int l = 0; l < nlev; l++) {
+      lp[l] = s;
+      for (int t = 0; t < T; t++) {
+        int &c = cnt[(size_t)t * (nlev + 1) + l];
+        const int k = c;
+        c = s;
+        s += k;
+      }
+    }
+    lp[nlev] = s;
   }
+  hs.slot_row.resize(n);
+  slot_of.resize(n);
+  par_run(T, [&](int t) {
+    int *c = &cnt[(size_t)t * (nlev + 1)];
+    for (int i = (int)((long long)n * t / T); i < (int)((long long)n * (t + 1) / T); i++) {
+      const int s = c[lev[i]]++;
+      hs.slot_row[s] = i;
+      slot_of[i] = s;
+    }
+  });
   hs.task_ptr.clear();
   for (int l = 0; l < nlev; l++) {
-    for (int s = cnt[l]; s < cnt[l + 1]; s += kTaskRows) hs.task_ptr.push_back(s);
+    for (int s = lp[l]; s < lp[l + 1]; s += kTaskRows) hs.task_ptr.push_back(s);
   }
   hs.task_ptr.push_back(n);
   hs.nlevels = nlev;
-  hs.level_ptr.assign(cnt.begin(), cnt.end());
+  hs.level_ptr = std::move(lp);
 }
 
 // Inside every row, entries ordered by the level of the row they read,
-// latest last, ties by column, then by position (rows in parallel; short
-// rows by insertion sort, long ones by std::sort on local keys).
-// Single-column sweeps read a row from its end, so the lane that polls for
-// a wave watches the value produced last and, when it lands, the rest of
-// the row is usually there too.  Multi-column sweeps read from the start,
-// several rounds per row: the rounds over long-finished entries run while
-// the latest are still in flight.  The key fixes the order whatever order
-// the entries arrive in.
-void order_entries_by_level(HostSweep &hs, const std::vector<int> &lev) {
-  struct Ent {
-    unsigned long long key;  // level << 32 | column
-    unsigned pos;
-    double v;
-    bool operator<(const Ent &o) const { return key != o.key ? key < o.key : pos < o.pos; }
-  };
-  const size_t n = hs.slot_row.size();
-  par_for(n, [&](size_t s0, size_t s1) {
-    std::vector<Ent> buf;
-    for (size_t s = s0; s < s1; s++) {
-      const int e0 = hs.rp[s], e1 = hs.rp[s + 1];
-      const int k = e1 - e0;
-      if (k < 2) continue;
-      buf.resize(k);
-      bool sorted = true;
-      for (int t = 0; t < k; t++) {
-        const int j = hs.ci[e0 + t];
-        buf[t] = {((unsigned long long)(unsigned)lev[j] << 32) | (unsigned)j, (unsigned)t,
-                  hs.cv[e0 + t]};
-        if (t && buf[t] < buf[t - 1]) sorted = false;
-      }
-      if (sorted) continue;
-      if (k <= 16) {
-        for (int t = 1; t < k; t++) {
-          const Ent x = buf[t];
-          int u = t;
-          for (; u > 0 && x < buf[u - 1]; u--) buf[u] = buf[u - 1];
-          buf[u] = x;
-        }
-      } else {
-        std::sort(buf.begin(), buf.end());
-      }
-      for (int t = 0; t < k; t++) {
-        hs.ci[e0 + t] = (int)(unsigned)(buf[t].key & 0xffffffffu);
-        hs.cv[e0 + t] = buf[t].v;
-      }
+// latest last, ties by column, then by value.  Single-column sweeps read a row from its end, so
+// the lane that polls for a wave watches the value produced last and, when
+// it lands, the rest of the row is usually there too.  Multi-column sweeps
+// read from the start, several rounds per row: the rounds over
+// long-finished entries run while the latest are still in flight.  The key
+// fixes the order whatever order the entries arrive in.
+struct LevelEnt {
+  unsigned long long key;  // level << 32 | column
+  double v;
+  bool operator<(const LevelEnt &o) const {
+    if (key != o.key) return key < o.key;
+    unsigned long long a, b;
+    memcpy(&a, &v, 8);
+    memcpy(&b, &o.v, 8);
+    return a < b;
+  }
+};
+
+void sort_level_ents(LevelEnt *b, int k) {
+  // insertion sort: linear on the nearly sorted rows the analysis produces;
+  // std::sort takes over once it has moved more than 8 entries per entry
+  long long budget = 8LL * k + 64;
+  for (int t = 1; t < k; t++) {
+    if (!(b[t] < b[t - 1])) continue;
+    const LevelEnt x = b[t];
+    int u = t;
+    for (; u > 0 && x < b[u - 1]; u--) b[u] = b[u - 1];
+    b[u] = x;
+    budget -= t - u;
+    if (budget < 0) {
+      std::sort(b, b + k);
+      return;
     }
-  });
+  }
 }
 
 // Host analysis of the stored factor: validation (lower triangle, one
 // nonzero diagonal per column), dependency levels of both sweeps and their
 // slot-ordered CSR.  Forward: row i of G waits for every column j < i with
 // G(i,j) != 0.  Backward (G^T z = y): row i waits for every j > i with
-// G(j,i) != 0, i.e. the stored column i.  Host threads split the columns or
-// slots; the two level passes (sequential in the column order) run side by
-// side; the result does not depend on the thread count.
+// G(j,i) != 0, i.e. the stored column i.
+struct StagedEnt {
+  int i, j;
+  double v;
+};
+
 static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, const double *val,
                           HostSweep &hf, HostSweep &hb, int &nlf, int &nlb) {
   const size_t nnz = colPtr[n];
-  // Validate lower-triangular columns and find the diagonal of each; the
-  // error names the lowest bad column, as a sequential pass would.
-  std::vector<double> diag(n, 0.0);
-  std::vector<int> offf(n, 0);  // off-diagonals per row of G (forward sweep)
+  const size_t noff = nnz >= n ? nnz - n : 0;
+  const int T = (int)std::min<size_t>((size_t)host_threads(), std::max<size_t>(1, n / 4096));
+  const std::vector<size_t> cr = column_ranges(n, colPtr, T);
+  // pieces of 2^ps rows for the level passes (about 64 per thread) and
+  // buckets of 2^bs rows for the transposition (at most 256)
+  int ps = 10, bs = 10;
+  while (((size_t)64 * T << ps) < n) ps++;
+  while (((size_t)256 << bs) < n) bs++;
+  const int P = (int)((n + ((size_t)1 << ps) - 1) >> ps);
+  const int NB = (int)((n + ((size_t)1 << bs) - 1) >> bs);
+  const size_t words = ((size_t)P * P + 63) / 64;
+
+  // 1. columns: validation, pivots, entries per (thread, row bucket), and
+  // which pieces of columns feed which pieces of rows (a bitmap per thread)
+  auto diag_p = std::make_shared<DblVec>(n);
+  DblVec &diag_v = *diag_p;
+  std::vector<size_t> bcnt((size_t)T * NB, 0);
+  std::vector<unsigned long long> bm((size_t)T * words, 0);
+  size_t bad = n;
   auto column_ok = [&](size_t j) -> bool {
     if (colPtr[j + 1] < colPtr[j] || colPtr[j + 1] > nnz) return false;
     bool seen = false;
@@ -172,7 +353,32 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
     }
     return seen;
   };
-  auto report_column = [&](size_t j) -> int {
+  par_run(T, [&](int t) {
+    size_t *bc = &bcnt[(size_t)t * NB];
+    unsigned long long *w = &bm[(size_t)t * words];
+    for (size_t j = cr[t]; j < cr[t + 1]; j++) {
+      if (!column_ok(j)) {  // the lowest bad column of this range
+        size_t cur = __atomic_load_n(&bad, __ATOMIC_RELAXED);
+        while (j < cur &&
+               !__atomic_compare_exchange_n(&bad, &cur, j, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+        }
+        return;
+      }
+      const size_t base = (j >> ps) * (size_t)P;
+      for (size_t e = colPtr[j]; e < colPtr[j + 1]; e++) {
+        const size_t i = rowIdx[e];
+        if (i == j) {
+          diag_v[j] = val[e];
+          continue;
+        }
+        bc[i >> bs]++;
+        const size_t bit = base + (i >> ps);
+        w[bit >> 6] |= 1ull << (bit & 63);
+      }
+    }
+  });
+  if (bad < n) {  // the error names the lowest bad column, as a sequential pass would
+    const size_t j = bad;
     if (colPtr[j + 1] < colPtr[j]) return fail(RCHOL_AMD_EINVAL, "colPtr not monotone at %zu", j);
     if (colPtr[j + 1] > nnz)
       return fail(RCHOL_AMD_EINVAL, "colPtr[%zu] = %zu exceeds nnz = %zu", j + 1,
@@ -188,115 +394,196 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
       }
     }
     return fail(RCHOL_AMD_EPIVOT, "G(%zu,%zu) is zero or missing", j, j);
-  };
+  }
+
+  // 2. CSR of lower G in natural row order: entries staged by row bucket
+  // (bucket-major, then thread, then column order), then each bucket placed
+  // by row.  Inside a row the columns ascend.  The staging memory is the
+  // backward sweep's store (its entries are written in step 3).
+  std::vector<size_t> bst(NB + 1, 0);  // staged range of each bucket
+  std::vector<size_t> cur((size_t)T * NB);
   {
-    size_t bad = n;
-    par_for(n, [&](size_t j0, size_t j1) {
-      for (size_t j = j0; j < j1; j++) {
-        if (column_ok(j)) continue;
-        size_t cur = __atomic_load_n(&bad, __ATOMIC_RELAXED);
-        while (j < cur &&
-               !__atomic_compare_exchange_n(&bad, &cur, j, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
-        }
-        break;
+    size_t s = 0;
+    for (int b = 0; b < NB; b++) {
+      bst[b] = s;
+      for (int t = 0; t < T; t++) {
+        cur[(size_t)t * NB + b] = s;
+        s += bcnt[(size_t)t * NB + b];
       }
-    });
-    if (bad < n) return report_column(bad);
-    par_for(n, [&](size_t j0, size_t j1) {
-      for (size_t j = j0; j < j1; j++) {
+    }
+    bst[NB] = s;
+  }
+  hf.nat_rp.resize(n + 1);
+  alloc_entries(hf, noff);
+  hb.store.resize(std::max<size_t>(noff * sizeof(StagedEnt), 8));
+  {
+    int *cin = hf.nat_ci;
+    double *cvn = hf.nat_cv;
+    int *rpn = hf.nat_rp.data();
+    StagedEnt *st = reinterpret_cast<StagedEnt *>(hb.store.data());
+    par_run(T, [&](int t) {
+      size_t *c = &cur[(size_t)t * NB];
+      for (size_t j = cr[t]; j < cr[t + 1]; j++) {
         for (size_t e = colPtr[j]; e < colPtr[j + 1]; e++) {
           const size_t i = rowIdx[e];
-          if (i == j)
-            diag[j] = val[e];
-          else
-            __atomic_fetch_add(&offf[i], 1, __ATOMIC_RELAXED);
+          if (i == j) continue;
+          st[c[i >> bs]++] = {(int)i, (int)j, val[e]};
         }
       }
     });
-  }
-  std::vector<int> levf(n, 0), levb(n, 0);
-  nlf = 0;
-  nlb = 0;
-  std::thread fwd_levels([&]() {
-    int m = 0;
-    for (size_t j = 0; j < n; j++) {
-      const int lj = levf[j] + 1;
-      m = std::max(m, lj);
-      for (size_t e = colPtr[j]; e < colPtr[j + 1]; e++) {
-        size_t i = rowIdx[e];
-        if (i != j && levf[i] < lj) levf[i] = lj;
-      }
-    }
-    nlf = m;
-  });
-  for (size_t jj = n; jj-- > 0;) {
-    int m = 0;
-    for (size_t e = colPtr[jj]; e < colPtr[jj + 1]; e++) {
-      size_t i = rowIdx[e];
-      if (i != jj) m = std::max(m, levb[i] + 1);
-    }
-    levb[jj] = m;
-    nlb = std::max(nlb, m + 1);
-  }
-  fwd_levels.join();
-  const size_t noff = nnz - n;
-  std::vector<int> slot_of_f, slot_of_b;
-  {
-    std::thread fwd_slots([&]() { build_slots(levf, nlf, hf, slot_of_f); });
-    build_slots(levb, nlb, hb, slot_of_b);
-    fwd_slots.join();
-  }
-  // forward sweep: CSR of lower G in slot order.  A thread owns a range of
-  // rows and scans the columns that can reach it (j < its last row), so
-  // every row is written by one thread, in column order.
-  hf.rp.assign(n + 1, 0);
-  for (size_t s = 0; s < n; s++) hf.rp[s + 1] = hf.rp[s] + offf[hf.slot_row[s]];
-  hf.ci.resize(noff);
-  hf.cv.resize(noff);
-  hf.diag.assign(n, 0.0);
-  {
-    std::vector<int> cur(n);
-    par_for(n, [&](size_t i0, size_t i1) {
-      for (size_t i = i0; i < i1; i++) cur[i] = hf.rp[slot_of_f[i]];
-      for (size_t j = 0; j < i1; j++) {
-        for (size_t e = colPtr[j]; e < colPtr[j + 1]; e++) {
-          const size_t i = rowIdx[e];
-          if (i < i0 || i >= i1 || i == j) continue;
-          const int pos = cur[i]++;
-          hf.ci[pos] = (int)j;
-          hf.cv[pos] = val[e];
+    std::atomic<int> nextb{0};
+    par_run(std::min(T, NB), [&](int) {
+      std::vector<int> cnt((size_t)1 << bs);
+      for (int b; (b = nextb.fetch_add(1, std::memory_order_relaxed)) < NB;) {
+        const size_t r0 = (size_t)b << bs, r1 = std::min(n, r0 + ((size_t)1 << bs));
+        std::fill(cnt.begin(), cnt.begin() + (r1 - r0), 0);
+        for (size_t k = bst[b]; k < bst[b + 1]; k++) cnt[st[k].i - r0]++;
+        size_t s = bst[b];
+        for (size_t r = r0; r < r1; r++) {
+          rpn[r] = (int)s;
+          const int c = cnt[r - r0];
+          cnt[r - r0] = (int)s;
+          s += c;
+        }
+        for (size_t k = bst[b]; k < bst[b + 1]; k++) {
+          const int pos = cnt[st[k].i - r0]++;
+          cin[pos] = st[k].j;
+          cvn[pos] = st[k].v;
         }
       }
     });
-    par_for(n, [&](size_t s0, size_t s1) {
-      for (size_t s = s0; s < s1; s++) hf.diag[s] = diag[hf.slot_row[s]];
+    rpn[n] = (int)noff;
+  }
+  // the backward sweep's rows are the stored columns: column j's
+  // off-diagonal entries at colPtr[j] - j
+  static_assert(sizeof(StagedEnt) >= sizeof(int) + sizeof(double), "staging too small");
+  {
+    const size_t ci_bytes = (noff * sizeof(int) + 7) & ~(size_t)7;
+    hb.nat_ci = reinterpret_cast<int *>(hb.store.data());
+    hb.nat_cv = reinterpret_cast<double *>(hb.store.data() + ci_bytes);
+    hb.nat_rp.resize(n + 1);
+    par_for(n + 1, [&](size_t j0, size_t j1) {
+      for (size_t j = j0; j < j1; j++) hb.nat_rp[j] = (int)(colPtr[j] - j);
     });
   }
-  // backward sweep: the stored rows (CSR of G^T) in slot order
-  hb.rp.assign(n + 1, 0);
-  for (size_t s = 0; s < n; s++) {
-    int row = hb.slot_row[s];
-    hb.rp[s + 1] = hb.rp[s] + (int)(colPtr[row + 1] - colPtr[row] - 1);
-  }
-  hb.ci.resize(noff);
-  hb.cv.resize(noff);
-  hb.diag.assign(n, 0.0);
-  par_for(n, [&](size_t s0, size_t s1) {
-    for (size_t s = s0; s < s1; s++) {
-      int row = hb.slot_row[s];
-      int pos = hb.rp[s];
-      for (size_t e = colPtr[row]; e < colPtr[row + 1]; e++) {
-        size_t j = rowIdx[e];
-        if (j == (size_t)row) continue;
-        hb.ci[pos] = (int)j;
-        hb.cv[pos] = val[e];
-        pos++;
+
+  // 3. levels, and each row's entries sorted by level (its reads are
+  // final when its own level is known).  Forward: lev(i) = 1 + max lev(j)
+  // over row i (0 if empty), rows ascending, sorted in place; backward:
+  // lev(j) = 1 + max lev(i) over column j, columns descending, sorted into
+  // the backward store.  Task q < P is forward piece q, task P + q backward
+  // piece q; an entry from column piece a to row piece b > a makes forward b
+  // wait for forward a and backward a wait for backward b.
+  IntVec levf(n), levb(n);
+  std::vector<int> topf(T, 0), topb(T, 0);
+  {
+    for (int t = 1; t < T; t++)
+      for (size_t k = 0; k < words; k++) bm[k] |= bm[(size_t)t * words + k];
+    std::vector<int> deg_out(2 * P + 1, 0), indeg(2 * P, 0);
+    auto each_edge = [&](auto &&f) {
+      for (int a = 0; a < P; a++)
+        for (int b = a + 1; b < P; b++) {
+          const size_t bit = (size_t)a * P + b;
+          if (bm[bit >> 6] >> (bit & 63) & 1) f(a, b);
+        }
+    };
+    each_edge([&](int a, int b) {
+      deg_out[a + 1]++;      // forward a -> b
+      deg_out[P + b + 1]++;  // backward b -> a
+      indeg[b]++;
+      indeg[P + a]++;
+    });
+    for (int v = 0; v < 2 * P; v++) deg_out[v + 1] += deg_out[v];
+    std::vector<int> succ(deg_out[2 * P]), at(deg_out.begin(), deg_out.end() - 1);
+    each_edge([&](int a, int b) {
+      succ[at[a]++] = b;
+      succ[at[P + b]++] = P + a;
+    });
+    const int *rpn = hf.nat_rp.data();
+    int *cin = hf.nat_ci, *bci = hb.nat_ci;
+    double *cvn = hf.nat_cv, *bcv = hb.nat_cv;
+    run_dag(2 * P, deg_out, succ, indeg, T, [&](int t, int v) {
+      std::vector<LevelEnt> buf;  // one per task: no cache line shared between threads
+      int top = 0;
+      if (v < P) {
+        const size_t i0 = (size_t)v << ps, i1 = std::min(n, i0 + ((size_t)1 << ps));
+        for (size_t i = i0; i < i1; i++) {
+          const int e0 = rpn[i], k = rpn[i + 1] - e0;
+          buf.resize(k);
+          int m = 0;
+          for (int u = 0; u < k; u++) {
+            const int j = cin[e0 + u];
+            const int l = levf[j];
+            m = std::max(m, l + 1);
+            buf[u] = {((unsigned long long)(unsigned)l << 32) | (unsigned)j, cvn[e0 + u]};
+          }
+          levf[i] = m;
+          top = std::max(top, m);
+          sort_level_ents(buf.data(), k);
+          for (int u = 0; u < k; u++) {
+            cin[e0 + u] = (int)(unsigned)(buf[u].key & 0xffffffffu);
+            cvn[e0 + u] = buf[u].v;
+          }
+        }
+        topf[t] = std::max(topf[t], top);
+      } else {
+        // a column is read from its end: later rows finish earlier here
+        const size_t j0 = (size_t)(v - P) << ps, j1 = std::min(n, j0 + ((size_t)1 << ps));
+        for (size_t j = j1; j-- > j0;) {
+          buf.clear();
+          int m = 0;
+          for (size_t e = colPtr[j + 1]; e-- > colPtr[j];) {
+            const size_t i = rowIdx[e];
+            if (i == j) continue;
+            const int l = levb[i];
+            m = std::max(m, l + 1);
+            buf.push_back({((unsigned long long)(unsigned)l << 32) | (unsigned)i, val[e]});
+          }
+          levb[j] = m;
+          top = std::max(top, m);
+          const int k = (int)buf.size();
+          sort_level_ents(buf.data(), k);
+          const size_t o = colPtr[j] - j;
+          for (int u = 0; u < k; u++) {
+            bci[o + u] = (int)(unsigned)(buf[u].key & 0xffffffffu);
+            bcv[o + u] = buf[u].v;
+          }
+        }
+        topb[t] = std::max(topb[t], top);
       }
-      hb.diag[s] = diag[row];
-    }
-  });
-  order_entries_by_level(hf, levf);
-  order_entries_by_level(hb, levb);
+    });
+  }
+  nlf = *std::max_element(topf.begin(), topf.end()) + 1;
+  nlb = *std::max_element(topb.begin(), topb.end()) + 1;
+
+  // 4. slots, and each slot's entry range in the slot-order layout
+  IntVec slot_of_f, slot_of_b;
+  build_slots(levf, nlf, hf, slot_of_f);
+  build_slots(levb, nlb, hb, slot_of_b);
+  for (HostSweep *h : {&hf, &hb}) {
+    const IntVec &nr = h->nat_rp;
+    const IntVec &sr = h->slot_row;
+    par_prefix(n, [&](size_t s) { return nr[sr[s] + 1] - nr[sr[s]]; }, h->rp);
+    h->nat_diag = diag_p;
+  }
   return RCHOL_AMD_OK;
 }
 
+// The slot-order copy of a sweep's entries and pivots on the host (the
+// device builds its own from the natural-order store).
+void materialize_slots(HostSweep &h) {
+  const size_t n = h.slot_row.size();
+  const size_t m = (size_t)h.rp[n];
+  h.ci.resize(m);
+  h.cv.resize(m);
+  h.diag.resize(n);
+  par_for(n, [&](size_t s0, size_t s1) {
+    for (size_t s = s0; s < s1; s++) {
+      const int r = h.slot_row[s];
+      const int e0 = h.nat_rp[r], k = h.nat_rp[r + 1] - e0, o = h.rp[s];
+      h.diag[s] = (*h.nat_diag)[r];
+      std::copy(h.nat_ci + e0, h.nat_ci + e0 + k, &h.ci[o]);
+      std::copy(h.nat_cv + e0, h.nat_cv + e0 + k, &h.cv[o]);
+    }
+  });
+}

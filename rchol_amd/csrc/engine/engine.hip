@@ -22,6 +22,8 @@
 #include <thread>
 #include <vector>
 
+#include <sys/mman.h>
+
 #include "../../../include/rchol_amd.h"
 #include "kernels.hpp"
 #include "kernels_mr.hpp"
@@ -1608,6 +1610,58 @@ int upload_chunks(Sweep::Chunks &d, const HostChunks &h, int precision) {
   return RCHOL_AMD_OK;
 }
 
+// Slot-order layout of a sweep on the device from the host's natural-order
+// rows: slot s takes row slot_row[s] (one thread per slot).  The host built
+// rp from the same rows' lengths, so every copy stays inside both layouts.
+template <typename VT>
+__global__ void k_gather_slots(int n, const int *__restrict__ slot_row, const int *__restrict__ rp,
+                               const int *__restrict__ nat_rp, const int *__restrict__ nat_ci,
+                               const VT *__restrict__ nat_cv, const VT *__restrict__ nat_diag,
+                               int *__restrict__ ci, VT *__restrict__ cv, VT *__restrict__ diag) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
+    const int r = slot_row[s];
+    const int a = nat_rp[r], k = nat_rp[r + 1] - a, o = rp[s];
+    for (int u = 0; u < k; u++) {
+      ci[o + u] = nat_ci[a + u];
+      cv[o + u] = nat_cv[a + u];
+    }
+    diag[s] = nat_diag[r];
+  }
+}
+
+// Entries and pivots of one sweep to the device: the natural-order rows
+// (values rounded to VT on the host) to temporaries, gathered there into
+// slot order.  sw.slot_row, sw.rp and sw.ci are already in place.
+template <typename VT>
+int gather_slots(rchol_amd_ctx *c, Sweep &sw, const HostSweep &h, DevBuf<VT> &cv, DevBuf<VT> &dg) {
+  const size_t n = h.slot_row.size(), m = (size_t)h.rp[n];
+  DevBuf<int> nrp, nci;
+  DevBuf<VT> ncv, ndg;
+  HIP_TRY(nrp.upload(h.nat_rp.data(), n + 1));
+  HIP_TRY(nci.upload(h.nat_ci, m));
+  if constexpr (sizeof(VT) == sizeof(double)) {
+    HIP_TRY(ncv.upload(h.nat_cv, m));
+    HIP_TRY(ndg.upload(h.nat_diag->data(), n));
+  } else {
+    std::vector<VT, NoInit<VT>> f(m), d(n);
+    par_for(m, [&](size_t e0, size_t e1) {
+      for (size_t e = e0; e < e1; e++) f[e] = (VT)h.nat_cv[e];
+    });
+    for (size_t r = 0; r < n; r++) d[r] = (VT)(*h.nat_diag)[r];
+    HIP_TRY(ncv.upload(f.data(), m));
+    HIP_TRY(ndg.upload(d.data(), n));
+  }
+  HIP_TRY(cv.alloc(m));
+  HIP_TRY(dg.alloc(n));
+  const int grid = (int)std::min<size_t>((n + kBlock - 1) / kBlock, (size_t)c->ncu * 16);
+  hipLaunchKernelGGL(k_gather_slots<VT>, dim3(std::max(grid, 1)), dim3(kBlock), 0, c->stream, (int)n,
+                     sw.slot_row.p, sw.rp.p, nrp.p, nci.p, ncv.p, ndg.p, sw.ci.p, cv.p, dg.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RCHOL_AMD_OK;
+}
+
 // Dynamic LDS and grid of the chunked sweeps of one precision: as many
 // workgroups as fit the CUs (any grid completes: the chunk queue needs no
 // co-residency).
@@ -1632,7 +1686,7 @@ int chunk_launch_setup(rchol_amd_ctx *c, size_t lds) {
 
 // Plan (and, when chosen, upload) the chunked sweeps of a factor whose
 // analysis is in hf / hb.
-int setup_chunks(rchol_amd_ctx *c, size_t n, const HostSweep &hf, const HostSweep &hb, int nlf,
+int setup_chunks(rchol_amd_ctx *c, size_t n, HostSweep &hf, HostSweep &hb, int nlf,
                  int nlb, int precision) {
   c->fwd.ch.reset();
   c->bwd.ch.reset();
@@ -1648,6 +1702,8 @@ int setup_chunks(rchol_amd_ctx *c, size_t n, const HostSweep &hf, const HostSwee
     return c->chunk_mode == 1 ? fail(RCHOL_AMD_EHIP, "%d bytes of LDS per workgroup is too little "
                                      "for the chunked sweeps", c->lds_max)
                               : RCHOL_AMD_OK;
+  materialize_slots(hf);
+  materialize_slots(hb);
   std::vector<int> slot_of_f(n), slot_of_b(n);
   for (size_t s = 0; s < n; s++) {
     slot_of_f[hf.slot_row[s]] = (int)s;
@@ -1704,27 +1760,19 @@ int rchol_amd_upload_G(rchol_amd_ctx *c, size_t n, const size_t *colPtr, const s
     sw.nnz_off = noff;
     HIP_TRY(sw.task_ptr.upload(h.task_ptr.data(), h.task_ptr.size()));
     sw.level_ptr_h = h.level_ptr;
-    sw.rp_h = h.rp;
+    sw.rp_h.assign(h.rp.begin(), h.rp.end());
     HIP_TRY(sw.level_ptr.upload(h.level_ptr.data(), h.level_ptr.size()));
     HIP_TRY(sw.slot_row.upload(h.slot_row.data(), n));
     HIP_TRY(sw.rp.upload(h.rp.data(), n + 1));
-    HIP_TRY(sw.ci.upload(h.ci.data(), noff));
     sw.cv64.reset();
     sw.diag64.reset();
     sw.cv32.reset();
     sw.diag32.reset();
-    if (precision == RCHOL_AMD_FP32) {
-      std::vector<float, NoInit<float>> f(noff);
-      par_for(noff, [&](size_t e0, size_t e1) {
-        for (size_t e = e0; e < e1; e++) f[e] = (float)h.cv[e];
-      });
-      HIP_TRY(sw.cv32.upload(f.data(), noff));
-      std::vector<float> d(h.diag.begin(), h.diag.end());
-      HIP_TRY(sw.diag32.upload(d.data(), n));
-    } else {
-      HIP_TRY(sw.cv64.upload(h.cv.data(), noff));
-      HIP_TRY(sw.diag64.upload(h.diag.data(), n));
-    }
+    if (h.rp[n] != (int)noff || h.nat_rp[n] != (int)noff)
+      return fail(RCHOL_AMD_EINVAL, "factor analysis lost entries");
+    HIP_TRY(sw.ci.alloc(noff));
+    return precision == RCHOL_AMD_FP32 ? gather_slots<float>(c, sw, h, sw.cv32, sw.diag32)
+                                       : gather_slots<double>(c, sw, h, sw.cv64, sw.diag64);
     return RCHOL_AMD_OK;
   };
   int rc = up(c->fwd, hf);
@@ -2430,6 +2478,8 @@ int rchol_amd_chunk_apply_host(size_t n, const size_t *colPtr, const size_t *row
   int nlf = 0, nlb = 0;
   int rc = analyze_factor(n, colPtr, rowIdx, val, hf, hb, nlf, nlb);
   if (rc) return rc;
+  materialize_slots(hf);
+  materialize_slots(hb);
   std::vector<int> slot_of_f(n), slot_of_b(n);
   for (size_t s = 0; s < n; s++) {
     slot_of_f[hf.slot_row[s]] = (int)s;
