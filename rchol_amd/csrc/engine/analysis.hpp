@@ -13,8 +13,8 @@
 //   2. the CSR of lower G in natural row order (a transposition through
 //      row buckets, with no atomic operations);
 //   3. levels of both sweeps over pieces of rows, each piece a task that
-//      runs once every piece it reads is done; each row's entries sorted
-//      for its sweep as its level is found;
+//      runs once every piece it reads is done; then each row's entries
+//      sorted for its sweep;
 //   4. slots (stable counting sort by level).
 // Rows stay in natural order, each sorted for its sweep in step 3; the
 // device gathers them into slot order (materialize_slots() does it on the
@@ -272,12 +272,12 @@ void build_slots(const IntVec &lev, int nlev, HostSweep &hs, IntVec &slot_of) {
 }
 
 // Inside every row, entries ordered by the level of the row they read,
-// latest last, ties by column, then by value.  Single-column sweeps read a row from its end, so
-// the lane that polls for a wave watches the value produced last and, when
-// it lands, the rest of the row is usually there too.  Multi-column sweeps
-// read from the start, several rounds per row: the rounds over
-// long-finished entries run while the latest are still in flight.  The key
-// fixes the order whatever order the entries arrive in.
+// latest last, ties by column, then by value.  Single-column sweeps read a
+// row from its end, so the lane that polls for a wave watches the value
+// produced last and, when it lands, the rest of the row is usually there
+// too.  Multi-column sweeps read from the start, several rounds per row: the
+// rounds over long-finished entries run while the latest are still in
+// flight.  The key fixes the order whatever order the entries arrive in.
 struct LevelEnt {
   unsigned long long key;  // level << 32 | column
   double v;
@@ -324,21 +324,16 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
   const size_t noff = nnz >= n ? nnz - n : 0;
   const int T = (int)std::min<size_t>((size_t)host_threads(), std::max<size_t>(1, n / 4096));
   const std::vector<size_t> cr = column_ranges(n, colPtr, T);
-  // pieces of 2^ps rows for the level passes (about 64 per thread) and
-  // buckets of 2^bs rows for the transposition (at most 256)
-  int ps = 10, bs = 10;
-  while (((size_t)64 * T << ps) < n) ps++;
-  while (((size_t)256 << bs) < n) bs++;
-  const int P = (int)((n + ((size_t)1 << ps) - 1) >> ps);
+  // buckets of 2^bs rows for the transposition: 1024 rows (at most 65536
+  // buckets), so that a bucket's rows stay in a core's L2 while placed
+  int bs = 10;
+  while (((size_t)65536 << bs) < n) bs++;
   const int NB = (int)((n + ((size_t)1 << bs) - 1) >> bs);
-  const size_t words = ((size_t)P * P + 63) / 64;
 
-  // 1. columns: validation, pivots, entries per (thread, row bucket), and
-  // which pieces of columns feed which pieces of rows (a bitmap per thread)
+  // 1. columns: validation, pivots, entries per (thread, row bucket)
   auto diag_p = std::make_shared<DblVec>(n);
   DblVec &diag_v = *diag_p;
   std::vector<size_t> bcnt((size_t)T * NB, 0);
-  std::vector<unsigned long long> bm((size_t)T * words, 0);
   size_t bad = n;
   auto column_ok = [&](size_t j) -> bool {
     if (colPtr[j + 1] < colPtr[j] || colPtr[j + 1] > nnz) return false;
@@ -355,7 +350,6 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
   };
   par_run(T, [&](int t) {
     size_t *bc = &bcnt[(size_t)t * NB];
-    unsigned long long *w = &bm[(size_t)t * words];
     for (size_t j = cr[t]; j < cr[t + 1]; j++) {
       if (!column_ok(j)) {  // the lowest bad column of this range
         size_t cur = __atomic_load_n(&bad, __ATOMIC_RELAXED);
@@ -364,7 +358,6 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
         }
         return;
       }
-      const size_t base = (j >> ps) * (size_t)P;
       for (size_t e = colPtr[j]; e < colPtr[j + 1]; e++) {
         const size_t i = rowIdx[e];
         if (i == j) {
@@ -372,8 +365,6 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
           continue;
         }
         bc[i >> bs]++;
-        const size_t bit = base + (i >> ps);
-        w[bit >> 6] |= 1ull << (bit & 63);
       }
     }
   });
@@ -467,13 +458,81 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
     });
   }
 
-  // 3. levels, and each row's entries sorted by level (its reads are
-  // final when its own level is known).  Forward: lev(i) = 1 + max lev(j)
-  // over row i (0 if empty), rows ascending, sorted in place; backward:
-  // lev(j) = 1 + max lev(i) over column j, columns descending, sorted into
-  // the backward store.  Task q < P is forward piece q, task P + q backward
-  // piece q; an entry from column piece a to row piece b > a makes forward b
-  // wait for forward a and backward a wait for backward b.
+  // Pieces for the level passes: runs of rows of about equal work (both
+  // sweeps' entries of a row, plus 2), about 64 per thread, so a dense
+  // separator spreads over several pieces.  A piece also ends where a run
+  // of rows that read nothing begins: a nested-dissection block starts so
+  // (its first independent set, or its first row in the given order), and a
+  // piece across two blocks would chain them.  A bitmap per thread marks
+  // which pieces of columns feed which pieces of rows.
+  constexpr int kMaxPieces = 4096;
+  IntVec piece_of;
+  std::vector<size_t> pstart;
+  int P = 0;
+  std::vector<unsigned long long> bm;
+  size_t words = 0;
+  {
+    const IntVec &nr = hf.nat_rp;
+    auto weight = [&](size_t r) {
+      return (size_t)(nr[r + 1] - nr[r]) + (size_t)(colPtr[r + 1] - colPtr[r] - 1) + 2;
+    };
+    auto reads = [&](size_t r) { return nr[r + 1] > nr[r]; };
+    std::vector<size_t> sum(T + 1, 0);
+    par_run(T, [&](int t) {
+      size_t w = 0;
+      for (size_t r = n * t / T; r < n * (t + 1) / T; r++) w += weight(r);
+      sum[t + 1] = w;
+    });
+    for (int t = 0; t < T; t++) sum[t + 1] += sum[t];
+    const size_t target = std::max<size_t>(1, sum[T] / (64 * (size_t)T));
+    IntVec cut(n), first;  // first[r] = cuts in rows [0, r)
+    for (int blocks = 1; blocks >= 0; blocks--) {
+      par_run(T, [&](int t) {
+        size_t w = sum[t];  // work of the rows before r
+        for (size_t r = n * t / T; r < n * (t + 1) / T; r++) {
+          bool c = r == 0;
+          if (!c) {
+            const size_t wp = w - weight(r - 1);
+            c = w / target != wp / target || (blocks && !reads(r) && reads(r - 1));
+          }
+          cut[r] = c ? 1 : 0;
+          w += weight(r);
+        }
+      });
+      par_prefix(n, [&](size_t r) { return cut[r]; }, first);
+      P = first[n];
+      if (P <= kMaxPieces) break;  // else too many block starts: by work only
+    }
+    pstart.assign(P + 1, n);
+    piece_of.resize(n);
+    par_for(n, [&](size_t r0, size_t r1) {
+      for (size_t r = r0; r < r1; r++) {
+        piece_of[r] = first[r + 1] - 1;
+        if (cut[r]) pstart[first[r]] = r;
+      }
+    });
+    words = ((size_t)P * P + 63) / 64;
+    bm.assign((size_t)T * words, 0);
+    par_run(T, [&](int t) {
+      unsigned long long *w = &bm[(size_t)t * words];
+      for (size_t i = n * t / T; i < n * (t + 1) / T; i++) {
+        const size_t pi = (size_t)piece_of[i];
+        for (int e = nr[i]; e < nr[i + 1]; e++) {
+          const size_t pj = (size_t)piece_of[hf.nat_ci[e]];
+          if (pj == pi) continue;
+          const size_t bit = pj * P + pi;
+          w[bit >> 6] |= 1ull << (bit & 63);
+        }
+      }
+    });
+  }
+
+  // 3. levels, then each row's entries sorted by level.  Forward: lev(i) =
+  // 1 + max lev(j) over row i (0 if empty), rows ascending; backward: lev(j)
+  // = 1 + max lev(i) over column j, columns descending.  Task q < P is
+  // forward piece q, task P + q backward piece q; an entry from column
+  // piece a to row piece b > a makes forward b wait for forward a and
+  // backward a wait for backward b.
   IntVec levf(n), levb(n);
   std::vector<int> topf(T, 0), topb(T, 0);
   {
@@ -503,44 +562,67 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
     int *cin = hf.nat_ci, *bci = hb.nat_ci;
     double *cvn = hf.nat_cv, *bcv = hb.nat_cv;
     run_dag(2 * P, deg_out, succ, indeg, T, [&](int t, int v) {
-      std::vector<LevelEnt> buf;  // one per task: no cache line shared between threads
       int top = 0;
       if (v < P) {
-        const size_t i0 = (size_t)v << ps, i1 = std::min(n, i0 + ((size_t)1 << ps));
-        for (size_t i = i0; i < i1; i++) {
-          const int e0 = rpn[i], k = rpn[i + 1] - e0;
-          buf.resize(k);
+        for (size_t i = pstart[v]; i < pstart[v + 1]; i++) {
           int m = 0;
-          for (int u = 0; u < k; u++) {
-            const int j = cin[e0 + u];
-            const int l = levf[j];
-            m = std::max(m, l + 1);
-            buf[u] = {((unsigned long long)(unsigned)l << 32) | (unsigned)j, cvn[e0 + u]};
-          }
+          for (int e = rpn[i]; e < rpn[i + 1]; e++) m = std::max(m, levf[cin[e]] + 1);
           levf[i] = m;
           top = std::max(top, m);
+        }
+        topf[t] = std::max(topf[t], top);
+      } else {
+        for (size_t j = pstart[v - P + 1]; j-- > pstart[v - P];) {
+          int m = 0;
+          for (size_t e = colPtr[j]; e < colPtr[j + 1]; e++) {
+            const size_t i = rowIdx[e];
+            if (i != j) m = std::max(m, levb[i] + 1);
+          }
+          levb[j] = m;
+          top = std::max(top, m);
+        }
+        topb[t] = std::max(topb[t], top);
+      }
+    });
+    // Every row's entries in sweep order, rows in parallel (chunks taken
+    // from a shared counter: the dense separator rows come last).  Forward
+    // rows are sorted in place, and most already are; backward rows (the
+    // stored columns) are read from their ends, since later rows finish
+    // earlier in that sweep, and written to the backward store.
+    std::atomic<size_t> next{0};
+    constexpr size_t kRowChunk = 4096;
+    par_run(T, [&](int) {
+      std::vector<LevelEnt> buf;
+      for (size_t r0; (r0 = next.fetch_add(kRowChunk, std::memory_order_relaxed)) < n;) {
+        const size_t r1 = std::min(n, r0 + kRowChunk);
+        for (size_t i = r0; i < r1; i++) {
+          const int e0 = rpn[i], k = rpn[i + 1] - e0;
+          bool sorted = true;
+          unsigned long long prev = 0;
+          for (int u = 0; u < k && sorted; u++) {
+            const int j = cin[e0 + u];
+            const unsigned long long key = ((unsigned long long)(unsigned)levf[j] << 32) | (unsigned)j;
+            sorted = u == 0 || key > prev;
+            prev = key;
+          }
+          if (sorted) continue;
+          buf.resize(k);
+          for (int u = 0; u < k; u++) {
+            const int j = cin[e0 + u];
+            buf[u] = {((unsigned long long)(unsigned)levf[j] << 32) | (unsigned)j, cvn[e0 + u]};
+          }
           sort_level_ents(buf.data(), k);
           for (int u = 0; u < k; u++) {
             cin[e0 + u] = (int)(unsigned)(buf[u].key & 0xffffffffu);
             cvn[e0 + u] = buf[u].v;
           }
         }
-        topf[t] = std::max(topf[t], top);
-      } else {
-        // a column is read from its end: later rows finish earlier here
-        const size_t j0 = (size_t)(v - P) << ps, j1 = std::min(n, j0 + ((size_t)1 << ps));
-        for (size_t j = j1; j-- > j0;) {
+        for (size_t j = r0; j < r1; j++) {
           buf.clear();
-          int m = 0;
           for (size_t e = colPtr[j + 1]; e-- > colPtr[j];) {
             const size_t i = rowIdx[e];
-            if (i == j) continue;
-            const int l = levb[i];
-            m = std::max(m, l + 1);
-            buf.push_back({((unsigned long long)(unsigned)l << 32) | (unsigned)i, val[e]});
+            if (i != j) buf.push_back({((unsigned long long)(unsigned)levb[i] << 32) | (unsigned)i, val[e]});
           }
-          levb[j] = m;
-          top = std::max(top, m);
           const int k = (int)buf.size();
           sort_level_ents(buf.data(), k);
           const size_t o = colPtr[j] - j;
@@ -549,7 +631,6 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
             bcv[o + u] = buf[u].v;
           }
         }
-        topb[t] = std::max(topb[t], top);
       }
     });
   }
