@@ -1,7 +1,9 @@
 """Host factorization, orderings and generators (librchol_factor.so) -- CPU.
 
 Pins: bit-identity with the reference's own factorization on the golden
-fixtures (and live against oracle/_ref where it is built), unbiasedness of
+fixtures (and live against oracle/_ref where it is built), the MIS order's
+factor against the reference's elimination on the order MIS chose,
+unbiasedness of
 the sampled factor (E[G G^T] = A), the 1D closed form, and structural
 invariants of both orderings.
 """
@@ -75,6 +77,39 @@ def test_static_factor_matches_reference_live(case):
         g, _, _ = F.factor(A, perm, blocks, "static", 1)
     for a, b in zip(g, g_ref):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("case", ["var12", "var16", "var2d64"])
+def test_mis_factor_is_the_reference_algorithm_on_its_own_order(case):
+    """The MIS order picks the elimination order inside each block while it
+    eliminates; the factor it returns is then the reference's elimination
+    (rchol_lap, built from its sources when present, else this package's
+    bit-identical static path) run on that same order and seed: the same
+    sampled pattern, values equal up to the order in which duplicate fill is
+    summed (1e-12 relative).  Variable coefficients: with the constant
+    Laplacian's tied weights that summation order can reorder a tie and so
+    the sampling."""
+    if case == "var12":
+        A, grid, depth = F.varcoef_3d(12, 12, 12, 1.5, 1e-3, 3), (12, 12, 12), 3
+    elif case == "var16":
+        A, grid, depth = F.varcoef_3d(16, 16, 16, 1.5, 1e-3, 8), (16, 16, 16), 3
+    else:
+        A, grid, depth = F.varcoef_3d(64, 64, 1, 1.0, 0.0, 5), (64, 64, 1), 4
+    n = A.shape[0]
+    perm, blocks = F.nd_grid(*grid, depth)
+    Gm, po, _ = F.factor(A, perm, blocks, "mis", 7)
+    if O.ref_available():
+        Gr = O.ref_rchol_nd(A, po, blocks, 7)
+    else:
+        Gr, _, _ = F.factor(A, po, blocks, "static", 7)
+
+    def mat(G):
+        cp, ri, gv = G
+        return sp.csc_matrix((gv, ri.astype(np.int64), cp.astype(np.int64)), shape=(n, n))
+    Mm, Mr = mat(Gm), mat(Gr)
+    assert Mm.nnz == Mr.nnz
+    assert (Mm != 0).multiply(Mr != 0).sum() == Mm.nnz          # same pattern
+    assert abs(Mm - Mr).max() <= 1e-12 * abs(Mr).max()
 
 
 def test_laplace_3d_matches_reference_closed_form():

@@ -204,3 +204,40 @@ def test_reuse_partition_sequence_updates_values_in_place():
     assert i2 == i3 and r2 == r3 and np.array_equal(x2, x3)
     xo, ro, io = O.pcg(Bperm, b[p], 1e-6, 200, rchol_amd.stored_from_lower(L))
     assert abs(i2 - io) <= 1 and rel(x2, xo) <= 1e-9
+
+
+def test_set_perm_solves_in_the_original_numbering():
+    """rchol_amd_set_perm: with A(P,P) and its factor resident, pcg, precond
+    and the batched solve take and return vectors in the original numbering
+    (b(P) gathered, x(P) = x' scattered on the device), against the oracle
+    on the permuted system; clearing P restores the permuted numbering."""
+    print("case set_perm 20^3 ND 8 leaves", flush=True)
+    A = F.varcoef_3d(20, 20, 20, 1.5, 1e-3, 12)
+    n = A.shape[0]
+    perm, blocks = F.nd_grid(20, 20, 20, 4)
+    Gs, po, _ = F.factor(A, perm, blocks, "mis", 3)
+    P = po.astype(np.int64)
+    Ap = F.permute(A, po)
+    b = np.random.default_rng(21).uniform(-1, 1, n)
+    r = np.random.default_rng(22).uniform(-1, 1, n)
+    xo_p, _, io = O.pcg(Ap, b[P], 1e-10, 500, Gs)
+    zo = np.empty(n)
+    zo[P] = O.precond(Gs, r[P])
+    with rchol_amd.Solver(Ap, Gs, rchol_amd.FP64) as s:
+        s.set_perm(po)
+        x, relres, itr = s.pcg(b, 1e-10, 500)
+        z = s.precond(r)
+        B = np.stack([b, r, b - r, 2 * r], axis=1)
+        X, rr, it = s.pcg_batch(B, 1e-10, 500)
+        s.set_perm(None)
+        xp, _, _ = s.pcg(b[P], 1e-10, 500)
+    x_orig = np.empty(n)
+    x_orig[P] = xo_p
+    assert abs(itr - io) <= 1
+    assert rel(x, x_orig) <= 1e-8
+    assert rel(z, zo) <= 1e-12
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) <= 1e-9
+    for c in range(B.shape[1]):
+        assert np.linalg.norm(A @ X[:, c] - B[:, c]) / np.linalg.norm(B[:, c]) <= 1e-9
+    np.testing.assert_allclose(X[:, 0], x, rtol=0, atol=1e-9 * np.abs(x).max())
+    assert rel(xp, xo_p) <= 1e-8
