@@ -430,6 +430,11 @@ int rchol_factor(size_t n, const size_t *rowPtr, const size_t *colIdx, const dou
     });
     for (size_t v = 0; v < n; v++) out->perm[F.pos[v]] = p[v];
     out->seconds = std::chrono::duration<double>(t1 - t0).count();
+    // release the n column lists on all threads (one thread freeing them in
+    // F's destructor takes longer than the rest of the hand-off)
+    rchol_host::parallel_for(F.cols.size(), [&](size_t c0, size_t c1) {
+      for (size_t c = c0; c < c1; c++) std::vector<Ent>().swap(F.cols[c].e);
+    });
   } catch (const std::exception &ex) {
     return fail(-2, std::string("factorization failed: ") + ex.what());
   }

@@ -9,6 +9,7 @@ does) and, through factor(), as the stored triple of the C++ SparseCSR G.
 """
 import ctypes
 import math
+import weakref
 
 import numpy as np
 import scipy.sparse as sp
@@ -144,15 +145,21 @@ def factor(A, perm=None, blocks=None, order="static", seed=0):
                                     v.ctypes.data_as(_lib.f64p), pp,
                                     blocks.ctypes.data_as(_lib.u64p), blocks.size,
                                     _ORDERS[order], int(seed) & 0xFFFFFFFF, ctypes.byref(out)))
-    L = _lib.factor()
-    cp = np.ctypeslib.as_array(out.colPtr, shape=(n + 1,)).copy()
+    cp = _owned(out.colPtr, n + 1)
     nnz = int(cp[-1])
-    ri = np.ctypeslib.as_array(out.rowIdx, shape=(max(nnz, 1),))[:nnz].copy()
-    gv = np.ctypeslib.as_array(out.val, shape=(max(nnz, 1),))[:nnz].copy()
-    po = np.ctypeslib.as_array(out.perm, shape=(n,)).copy()
-    for p in (out.colPtr, out.rowIdx, out.val, out.perm):
-        L.rchol_factor_free(ctypes.cast(p, ctypes.c_void_p))
+    ri = _owned(out.rowIdx, nnz)
+    gv = _owned(out.val, nnz)
+    po = _owned(out.perm, n)
     return (cp, ri, gv), po, out.seconds
+
+
+def _owned(ptr, count):
+    """numpy view of count elements of a buffer the factorization malloc'ed,
+    released by rchol_factor_free once the last view of it is gone (no copy:
+    the factor's arrays run to gigabytes)."""
+    base = np.ctypeslib.as_array(ptr, shape=(max(count, 1),))
+    weakref.finalize(base, _lib.factor().rchol_factor_free, ctypes.cast(ptr, ctypes.c_void_p))
+    return base[:count]
 
 
 def lower(Gs):
