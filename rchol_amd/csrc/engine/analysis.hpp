@@ -17,8 +17,7 @@
 //      sorted for its sweep;
 //   4. slots (stable counting sort by level).
 // Rows stay in natural order, each sorted for its sweep in step 3; the
-// device gathers them into slot order (materialize_slots() does it on the
-// host for the chunk planner).
+// device gathers them into slot order.
 // The result does not depend on the thread count.
 #pragma once
 
@@ -78,8 +77,7 @@ using DblVec = std::vector<double, NoInit<double>>;
 // Host-side schedule of one sweep before upload.  The rows' entries are
 // kept in natural row order (row r at [nat_rp[r], nat_rp[r + 1]) of nat_ci /
 // nat_cv, each row already in sweep order, in `store`); the device gathers
-// them into slot order.  ci / cv / diag hold the slot-order copy on the
-// host only after materialize_slots() (the chunk planner reads it).
+// them into slot order, and the chunk planner reads them as they are.
 struct HostSweep {
   std::vector<int> task_ptr, level_ptr;
   IntVec slot_row, rp;  // slot s reads row slot_row[s]; its entries at [rp[s], rp[s + 1])
@@ -88,8 +86,6 @@ struct HostSweep {
   int *nat_ci = nullptr;
   double *nat_cv = nullptr;
   std::shared_ptr<const DblVec> nat_diag;  // pivots, natural order
-  IntVec ci;
-  DblVec cv, diag;
   int nlevels = 0;
 };
 
@@ -648,23 +644,4 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
     h->nat_diag = diag_p;
   }
   return RCHOL_AMD_OK;
-}
-
-// The slot-order copy of a sweep's entries and pivots on the host (the
-// device builds its own from the natural-order store).
-void materialize_slots(HostSweep &h) {
-  const size_t n = h.slot_row.size();
-  const size_t m = (size_t)h.rp[n];
-  h.ci.resize(m);
-  h.cv.resize(m);
-  h.diag.resize(n);
-  par_for(n, [&](size_t s0, size_t s1) {
-    for (size_t s = s0; s < s1; s++) {
-      const int r = h.slot_row[s];
-      const int e0 = h.nat_rp[r], k = h.nat_rp[r + 1] - e0, o = h.rp[s];
-      h.diag[s] = (*h.nat_diag)[r];
-      std::copy(h.nat_ci + e0, h.nat_ci + e0 + k, &h.ci[o]);
-      std::copy(h.nat_cv + e0, h.nat_cv + e0 + k, &h.cv[o]);
-    }
-  });
 }

@@ -1,9 +1,10 @@
 // Host plan of the chunked sweeps (k_sptrsv_chunk).  Included by engine.hip
 // inside its anonymous namespace, after HostSweep and par_for.
 //
-// Input: both sweeps' rows with their off-diagonal entries (HostSweep, rows
-// in level order, slot_of maps a row to its slot) and pivots.  The forward
-// sweep reads entries j < i, the backward sweep j > i.
+// Input: both sweeps' rows with their off-diagonal entries in natural row
+// order (HostSweep: row i's entries at [nat_rp[i], nat_rp[i + 1]) of
+// nat_ci / nat_cv, in sweep order) and pivots.  The forward sweep reads
+// entries j < i, the backward sweep j > i.
 
 // Bounds of the chunks: one partition of [0, n) into index ranges shared by
 // both sweeps.  A range ends at S rows, and, once it holds kChunkMinRows,
@@ -30,14 +31,12 @@ void par_each(size_t n, F &&f) {
 }
 constexpr int kChunkWindow = 64;
 
-std::vector<int> chunk_bounds(const HostSweep &hf, const std::vector<int> &slot_of, int n,
-                              int S) {
+std::vector<int> chunk_bounds(const HostSweep &hf, int n, int S) {
   std::vector<int> lo(n);
   par_for((size_t)n, [&](size_t i0, size_t i1) {
     for (size_t i = i0; i < i1; i++) {
-      const int s = slot_of[i];
       int m = (int)i;
-      for (int e = hf.rp[s]; e < hf.rp[s + 1]; e++) m = std::min(m, hf.ci[e]);
+      for (int e = hf.nat_rp[i]; e < hf.nat_rp[i + 1]; e++) m = std::min(m, hf.nat_ci[e]);
       lo[i] = m;
     }
   });
@@ -75,11 +74,11 @@ constexpr long long kExtPartEntries = 8192;
 struct HostChunks {
   int nchunks = 0, nparts = 0;
   std::vector<int> c_part0, p_chunk, p_l0, p_l1, p_lg;  // parts of the external sums
-  std::vector<int> x_rp;  // per chunk-local row (q): its first external entry
+  IntVec x_rp;  // per chunk-local row (q): its first external entry
   std::vector<int> order, q0, m, r0, r1, split;
   std::vector<int> dep_ptr, dep;  // the work items each work item waits for (CSR)
-  std::vector<int> q_row;
-  std::vector<double> q_diag;
+  IntVec q_row;
+  DblVec q_diag;
   std::vector<int, NoInit<int>> x_l, x_ci;
   std::vector<double, NoInit<double>> x_cv;
   std::vector<unsigned, NoInit<unsigned>> r_w;
@@ -130,20 +129,23 @@ constexpr double kChunkGatherUs = 2.0;
 
 // Plan of one sweep over the bounds b.  backward: the rows read j > i and
 // the chunks run from the last range down.
-void plan_chunks(const HostSweep &h, const std::vector<int> &slot_of, int n,
-                 const std::vector<int> &b, bool backward, HostChunks &out) {
+void plan_chunks(const HostSweep &h, int n, const std::vector<int> &b, bool backward,
+                 HostChunks &out) {
   const int C = (int)b.size() - 1;
   out.nchunks = C;
   out.q0.assign(C, 0);
   out.m.assign(C, 0);
   out.r0.assign(C, 0);
   out.r1.assign(C, 0);
-  std::vector<int> lidx(n), rounds_of(C, 0), ext_of(C, 0);
-  std::vector<int, NoInit<int>> ent_round(h.ci.size());
+  IntVec lidx(n);  // every row's entry is written in pass 1
+  std::vector<int> rounds_of(C, 0), ext_of(C, 0);
+  const int *hci = h.nat_ci;
+  const double *hcv = h.nat_cv;
+  const DblVec &hdiag = *h.nat_diag;
+  std::vector<int, NoInit<int>> ent_round((size_t)h.nat_rp[n]);
   auto row_entries = [&](int i, int &e0, int &e1) {
-    const int s = slot_of[i];
-    e0 = h.rp[s];
-    e1 = h.rp[s + 1];
+    e0 = h.nat_rp[i];
+    e1 = h.nat_rp[i + 1];
   };
   // pass 1: local order, local index, rounds of each internal entry
   par_each((size_t)C, [&](size_t c) {
@@ -157,7 +159,7 @@ void plan_chunks(const HostSweep &h, const std::vector<int> &slot_of, int n,
         int e0, e1, lev = 0;
         row_entries(i, e0, e1);
         for (int e = e0; e < e1; e++) {
-          const int j = h.ci[e];
+          const int j = hci[e];
           if (j >= a && j < z)
             lev = std::max(lev, llev[j - a] + 1);
           else
@@ -182,7 +184,7 @@ void plan_chunks(const HostSweep &h, const std::vector<int> &slot_of, int n,
         int e0, e1;
         row_entries(rows[l], e0, e1);
         for (int e = e0; e < e1; e++) {
-          const int j = h.ci[e];
+          const int j = hci[e];
           if (j < a || j >= z) continue;
           const int r = rf.take(fin[lidx[j]] + 1);
           ent_round[e] = r;
@@ -208,9 +210,9 @@ void plan_chunks(const HostSweep &h, const std::vector<int> &slot_of, int n,
     xbase[c + 1] = xbase[c] + ext_of[c];
   }
   out.rounds = rtot;
-  out.q_row.assign(n, 0);
-  out.q_diag.assign(n, 0.0);
-  out.x_rp.assign(n + 1, 0);
+  out.q_row.resize(n);  // pass 2 writes every row of every chunk
+  out.q_diag.resize(n);
+  out.x_rp.resize(n + 1);
   out.x_l.resize(xbase[C]);
   out.x_ci.resize(xbase[C]);
   out.x_cv.resize(xbase[C]);
@@ -240,24 +242,23 @@ void plan_chunks(const HostSweep &h, const std::vector<int> &slot_of, int n,
       long long xp = xbase[c];
       for (int l = 0; l < m; l++) {
         const int i = rows[l];
-        const int s = slot_of[i];
-        const double d = h.diag[s];
+        const double d = hdiag[i];
         out.q_row[a + l] = i;
         out.q_diag[a + l] = d;
         out.x_rp[a + l] = (int)xp;
-        for (int e = h.rp[s]; e < h.rp[s + 1]; e++) {
-          const int j = h.ci[e];
+        for (int e = h.nat_rp[i]; e < h.nat_rp[i + 1]; e++) {
+          const int j = hci[e];
           if (j < a || j >= z) {
             out.x_l[xp] = l;
             out.x_ci[xp] = j;
-            out.x_cv[xp] = h.cv[e];
+            out.x_cv[xp] = hcv[e];
             xp++;
             continue;
           }
           const int r = ent_round[e];
           const size_t slot = ((size_t)out.r0[c] + r) * 64 + lane[r]++;
           out.r_w[slot] = (unsigned)lidx[j] | ((unsigned)l << 16);
-          out.r_v[slot] = h.cv[e] / d;
+          out.r_v[slot] = hcv[e] / d;
         }
       }
       // a row-aligned, balanced split of the external entries over the threads
@@ -333,9 +334,10 @@ void plan_chunks(const HostSweep &h, const std::vector<int> &slot_of, int n,
   // other chunk for the chunks it reads.  Dequeue order: by depth in the
   // chunk graph, parts before their chunk, then in sweep order.  And the
   // critical path estimate.
-  std::vector<int> chunk_of(n);
-  for (int c = 0; c < C; c++)
-    for (int i = b[c]; i < b[c + 1]; i++) chunk_of[i] = c;
+  IntVec chunk_of(n);
+  par_each((size_t)C, [&](size_t c) {
+    for (int i = b[c]; i < b[c + 1]; i++) chunk_of[i] = (int)c;
+  });
   std::vector<int> depth(C, 0);
   std::vector<double> finish(C, 0.0);
   std::vector<std::vector<int>> deps(C);

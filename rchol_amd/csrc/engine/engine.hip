@@ -1702,17 +1702,10 @@ int setup_chunks(rchol_amd_ctx *c, size_t n, HostSweep &hf, HostSweep &hb, int n
     return c->chunk_mode == 1 ? fail(RCHOL_AMD_EHIP, "%d bytes of LDS per workgroup is too little "
                                      "for the chunked sweeps", c->lds_max)
                               : RCHOL_AMD_OK;
-  materialize_slots(hf);
-  materialize_slots(hb);
-  std::vector<int> slot_of_f(n), slot_of_b(n);
-  for (size_t s = 0; s < n; s++) {
-    slot_of_f[hf.slot_row[s]] = (int)s;
-    slot_of_b[hb.slot_row[s]] = (int)s;
-  }
-  const std::vector<int> bnd = chunk_bounds(hf, slot_of_f, (int)n, S);
+  const std::vector<int> bnd = chunk_bounds(hf, (int)n, S);
   HostChunks pf, pb;
-  plan_chunks(hf, slot_of_f, (int)n, bnd, false, pf);
-  plan_chunks(hb, slot_of_b, (int)n, bnd, true, pb);
+  plan_chunks(hf, (int)n, bnd, false, pf);
+  plan_chunks(hb, (int)n, bnd, true, pb);
   c->chunk_auto = pf.est_us + pb.est_us < (nlf + nlb) * kLevelUs;
   if (c->chunk_mode == -1 && !c->chunk_auto) return RCHOL_AMD_OK;
   int rc = upload_chunks(c->fwd.ch, pf, precision);
@@ -2478,17 +2471,10 @@ int rchol_amd_chunk_apply_host(size_t n, const size_t *colPtr, const size_t *row
   int nlf = 0, nlb = 0;
   int rc = analyze_factor(n, colPtr, rowIdx, val, hf, hb, nlf, nlb);
   if (rc) return rc;
-  materialize_slots(hf);
-  materialize_slots(hb);
-  std::vector<int> slot_of_f(n), slot_of_b(n);
-  for (size_t s = 0; s < n; s++) {
-    slot_of_f[hf.slot_row[s]] = (int)s;
-    slot_of_b[hb.slot_row[s]] = (int)s;
-  }
-  const std::vector<int> bnd = chunk_bounds(hf, slot_of_f, (int)n, max_rows);
+  const std::vector<int> bnd = chunk_bounds(hf, (int)n, max_rows);
   HostChunks pf, pb;
-  plan_chunks(hf, slot_of_f, (int)n, bnd, false, pf);
-  plan_chunks(hb, slot_of_b, (int)n, bnd, true, pb);
+  plan_chunks(hf, (int)n, bnd, false, pf);
+  plan_chunks(hb, (int)n, bnd, true, pb);
   std::vector<double> y(n);
   rc = simulate_chunks(pf, (int)n, r, y.data());
   if (!rc) rc = simulate_chunks(pb, (int)n, y.data(), z);
