@@ -346,8 +346,17 @@ int rchol_factor(size_t n, const size_t *rowPtr, const size_t *colIdx, const dou
     // (util.cpp:16-56 reorder + util.hpp:60-142 + rchol_lap.cpp:778-855).
     F.cols.resize(n + 1);
     F.diag.assign(n + 1, 0.0);
-    for (size_t k = 0; k < rowPtr[n]; k++)
-      if (colIdx[k] >= n) return fail(-1, "column index out of range");
+    {
+      std::atomic<bool> bad{false};
+      rchol_host::parallel_for(rowPtr[n], [&](size_t k0, size_t k1) {
+        for (size_t k = k0; k < k1; k++)
+          if (colIdx[k] >= n) {
+            bad.store(true, std::memory_order_relaxed);
+            return;
+          }
+      });
+      if (bad.load()) return fail(-1, "column index out of range");
+    }
     rchol_host::parallel_for(n, [&](size_t i0, size_t i1) {
       std::vector<Ent> row;
       for (size_t i = i0; i < i1; i++) {
@@ -358,6 +367,9 @@ int rchol_factor(size_t n, const size_t *rowPtr, const size_t *colIdx, const dou
         double sum = 0.0;
         for (const Ent &x : row) sum += x.w;
         Column &col = F.cols[i];
+        size_t keep = 1;  // the ground entry, if any
+        for (const Ent &x : row) keep += x.row >= i;
+        col.e.reserve(keep);
         for (const Ent &x : row)
           if (x.row >= i) col.e.push_back({x.row, x.w * -1});
         if (std::abs(sum) > 1e-09) {

@@ -28,32 +28,59 @@ int export_csr(const std::vector<size_t> &rp, const std::vector<size_t> &ci,
 
 // 7-point stencil assembly: w[axis][lower endpoint] = conductivity of the
 // face between idx and idx + stride(axis) (0 when outside); diag given.
+// A row's entry count follows from its position, so z-planes are filled in
+// parallel straight into the returned arrays.
 int assemble7(int nx, int ny, int nz, const std::vector<double> (&w)[3],
               const std::vector<double> &diag, size_t *n, size_t **rowPtr, size_t **colIdx,
               double **val) {
   const size_t N = (size_t)nx * ny * nz;
   const size_t sx = 1, sy = (size_t)nx, sz = (size_t)nx * ny;
-  std::vector<size_t> rp, ci;
-  std::vector<double> v;
-  rp.reserve(N + 1);
-  ci.reserve(7 * N);
-  v.reserve(7 * N);
-  rp.push_back(0);
-  for (int z = 0; z < nz; z++)
+  auto count = [&](int x, int y, int z) {
+    return (size_t)1 + (x > 0) + (x < nx - 1) + (y > 0) + (y < ny - 1) + (z > 0) + (z < nz - 1);
+  };
+  size_t *rp = (size_t *)malloc((N + 1) * sizeof(size_t));
+  if (!rp) return -2;
+  // entries before plane z: planes are alike except the first and the last
+  std::vector<size_t> plane(nz + 1, 0);
+  for (int z = 0; z < nz; z++) {
+    size_t c = 0;
     for (int y = 0; y < ny; y++)
-      for (int x = 0; x < nx; x++) {
-        const size_t i = x + y * sy + z * sz;
-        if (z > 0) { ci.push_back(i - sz); v.push_back(-w[2][i - sz]); }
-        if (y > 0) { ci.push_back(i - sy); v.push_back(-w[1][i - sy]); }
-        if (x > 0) { ci.push_back(i - sx); v.push_back(-w[0][i - sx]); }
-        ci.push_back(i);
-        v.push_back(diag[i]);
-        if (x < nx - 1) { ci.push_back(i + sx); v.push_back(-w[0][i]); }
-        if (y < ny - 1) { ci.push_back(i + sy); v.push_back(-w[1][i]); }
-        if (z < nz - 1) { ci.push_back(i + sz); v.push_back(-w[2][i]); }
-        rp.push_back(ci.size());
-      }
-  return export_csr(rp, ci, v, n, rowPtr, colIdx, val);
+      for (int x = 0; x < nx; x++) c += count(x, y, z);
+    plane[z + 1] = plane[z] + c;
+  }
+  const size_t nnz = plane[nz];
+  size_t *ci = (size_t *)malloc(std::max<size_t>(nnz, 1) * sizeof(size_t));
+  double *v = (double *)malloc(std::max<size_t>(nnz, 1) * sizeof(double));
+  if (!ci || !v) {
+    free(rp);
+    free(ci);
+    free(v);
+    return -2;
+  }
+  rchol_host::parallel_for((size_t)nz, [&](size_t z0, size_t z1) {
+    for (int z = (int)z0; z < (int)z1; z++) {
+      size_t k = plane[z];
+      for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++) {
+          const size_t i = x + y * sy + z * sz;
+          rp[i] = k;
+          if (z > 0) { ci[k] = i - sz; v[k++] = -w[2][i - sz]; }
+          if (y > 0) { ci[k] = i - sy; v[k++] = -w[1][i - sy]; }
+          if (x > 0) { ci[k] = i - sx; v[k++] = -w[0][i - sx]; }
+          ci[k] = i;
+          v[k++] = diag[i];
+          if (x < nx - 1) { ci[k] = i + sx; v[k++] = -w[0][i]; }
+          if (y < ny - 1) { ci[k] = i + sy; v[k++] = -w[1][i]; }
+          if (z < nz - 1) { ci[k] = i + sz; v[k++] = -w[2][i]; }
+        }
+    }
+  }, 1);
+  rp[N] = nnz;
+  *n = N;
+  *rowPtr = rp;
+  *colIdx = ci;
+  *val = v;
+  return 0;
 }
 
 }  // namespace
@@ -131,21 +158,40 @@ int rchol_grid_laplacian(int nx, int ny, int nz, const double *wt, double shift,
                          size_t **rowPtr, size_t **colIdx, double **val) {
   if (nx < 1 || ny < 1 || nz < 1 || !wt || !n || !rowPtr || !colIdx || !val) return -1;
   const size_t N = (size_t)nx * ny * nz;
-  const size_t st[3] = {1, (size_t)nx, (size_t)nx * ny};
-  const int ext[3] = {nx, ny, nz};
+  // edges in rchol_grid_edges order: axis by axis, each in index order of
+  // its lower endpoint; base[a] = the first edge of axis a
+  const size_t mx = (size_t)(nx - 1) * ny * nz, my = (size_t)nx * (ny - 1) * nz;
+  const size_t base[3] = {0, mx, mx + my};
   std::vector<double> w[3];
-  std::vector<double> diag(N, shift);
-  size_t e = 0;
-  for (int a = 0; a < 3; a++) {
-    w[a].assign(N, 0.0);
-    for (size_t i = 0; i < N; i++)
-      if ((int)((i / st[a]) % ext[a]) < ext[a] - 1) {
-        const double k = wt[e++];
-        w[a][i] = k;
-        diag[i] += k;
-        diag[i + st[a]] += k;
-      }
-  }
+  for (int a = 0; a < 3; a++) w[a].assign(N, 0.0);
+  std::vector<double> diag(N);
+  rchol_host::parallel_for((size_t)nz, [&](size_t z0, size_t z1) {
+    for (size_t z = z0; z < z1; z++)
+      for (size_t y = 0; y < (size_t)ny; y++)
+        for (size_t x = 0; x < (size_t)nx; x++) {
+          const size_t i = x + y * nx + z * nx * ny;
+          if (x < (size_t)nx - 1) w[0][i] = wt[base[0] + x + (nx - 1) * (y + ny * z)];
+          if (y < (size_t)ny - 1) w[1][i] = wt[base[1] + x + nx * (y + (ny - 1) * z)];
+          if (z < (size_t)nz - 1) w[2][i] = wt[base[2] + i];
+        }
+  }, 1);
+  // each row sums its own faces (no writes into other rows), in the order a
+  // sweep over the edges would add them: per axis the face below, then above
+  const size_t st[3] = {1, (size_t)nx, (size_t)nx * ny};
+  rchol_host::parallel_for((size_t)nz, [&](size_t z0, size_t z1) {
+    for (size_t z = z0; z < z1; z++)
+      for (size_t y = 0; y < (size_t)ny; y++)
+        for (size_t x = 0; x < (size_t)nx; x++) {
+          const size_t i = x + y * nx + z * nx * ny;
+          const size_t c[3] = {x, y, z};
+          double d = shift;
+          for (int a = 0; a < 3; a++) {
+            if (c[a] > 0) d += w[a][i - st[a]];
+            d += w[a][i];
+          }
+          diag[i] = d;
+        }
+  }, 1);
   return assemble7(nx, ny, nz, w, diag, n, rowPtr, colIdx, val);
 }
 

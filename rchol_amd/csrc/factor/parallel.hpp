@@ -12,10 +12,11 @@ inline unsigned worker_count() {
   return std::max(1u, std::min(t ? t : 1u, 16u));
 }
 
+// Below serial_below items the loop runs on the caller's thread.
 template <typename F>
-void parallel_for(size_t n, F fn) {
+void parallel_for(size_t n, F fn, size_t serial_below = 4096) {
   const unsigned T = worker_count();
-  if (T == 1 || n < 4096) {
+  if (T == 1 || n < serial_below) {
     fn((size_t)0, n);
     return;
   }

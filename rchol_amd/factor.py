@@ -28,15 +28,13 @@ def _chk(rc):
 
 
 def _take_csr(n, rp, ci, v):
-    L = _lib.factor()
     N = n.value
-    rpa = np.ctypeslib.as_array(rp, shape=(N + 1,)).copy()
+    rpa = _owned(rp, N + 1)
     nnz = int(rpa[-1])
-    cia = np.ctypeslib.as_array(ci, shape=(max(nnz, 1),))[:nnz].copy()
-    va = np.ctypeslib.as_array(v, shape=(max(nnz, 1),))[:nnz].copy()
-    for p in (rp, ci, v):
-        L.rchol_factor_free(ctypes.cast(p, ctypes.c_void_p))
-    return sp.csr_matrix((va, cia.astype(np.int64), rpa.astype(np.int64)), shape=(N, N))
+    # indices below 2^63: the int64 views share the buffers (scipy keeps or
+    # narrows them; the values are not copied)
+    return sp.csr_matrix((_owned(v, nnz), _owned(ci, nnz).view(np.int64), rpa.view(np.int64)),
+                         shape=(N, N))
 
 
 def laplace_3d(side):
