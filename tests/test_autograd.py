@@ -61,11 +61,12 @@ def dense_laplacian(dims, w, shift):
     return A, u, v
 
 
-def test_grid_laplacian_assembly_matches_dense():
-    dims = (4, 3, 2)
+@pytest.mark.parametrize("dims", [(4, 3, 2), (7, 6, 5), (5, 1, 3), (1, 4, 1), (1, 1, 1)])
+def test_grid_laplacian_assembly_matches_dense(dims):
     rng = np.random.default_rng(0)
     u, v = F.grid_edges(*dims)
-    assert u.size == 3 * 24 - 12 - 6 - 8
+    nx, ny, nz = dims
+    assert u.size == 3 * nx * ny * nz - nx * ny - ny * nz - nx * nz
     w = rng.uniform(0.5, 2.0, u.size)
     Ad, _, _ = dense_laplacian(dims, w, 0.3)
     A = F.grid_laplacian(*dims, w, 0.3)
