@@ -63,6 +63,12 @@ typedef struct rchol_amd_ctx rchol_amd_ctx;
                                          levels, used when its estimate beats the level
                                          schedule; 0: never; 1: always (planned at the next
                                          upload_G; set before it) */
+#define RCHOL_AMD_OPT_WAIT_TICKS 11   /* bound of every dependency / residency wait, in
+                                         ticks of the 100 MHz wall clock, for the GPU of
+                                         this context (every context on it); 0 restores
+                                         the default (2^26, ~0.67 s).  A lower bound turns
+                                         longer waits into RCHOL_AMD_ESTALL: for testing
+                                         the error path */
 
 /* Counters filled by rchol_amd_get_stats. */
 typedef struct rchol_amd_stats {

@@ -92,6 +92,9 @@ __device__ __forceinline__ T ld_stream(const T *p) {
 // broken schedule therefore ends as an error flag, never as a hang; once the
 // flag is up every other wait returns at its next check.
 constexpr long long kWaitTicks = 1ll << 26;
+// The bound every wait applies, per device (rchol_amd_set_option
+// RCHOL_AMD_OPT_WAIT_TICKS lowers it to force the error path in tests).
+__device__ long long g_wait_ticks = kWaitTicks;
 
 // Back-off cap of the polling lane of await_batch, in s_sleep(1) units
 // (64 clocks each): bounds the extra delay between publication and
@@ -109,7 +112,7 @@ constexpr int kPollCap = RCHOL_AMD_POLL_CAP;
 // remain, one lane polls the first pending address of the wave with
 // back-off, and once that value is there every lane re-reads all of its
 // pending slots at once (one load latency for the whole round).  A wait
-// longer than kWaitTicks raises the error flag; a raised flag zeroes what is
+// longer than g_wait_ticks raises the error flag; a raised flag zeroes what is
 // still pending and returns.
 // Slot u of a lane refers to x[j[u] * stride + col] (stride / col: the
 // interleaved columns of a multi-right-hand-side block; 1 / 0 otherwise).
@@ -142,7 +145,7 @@ __device__ __forceinline__ void await_batch(const T *x, const int (&j)[U], T (&x
         if (!Sent<T>::is(Sent<T>::load_agent(x + (size_t)jp * stride))) break;
         if (backoff < kPollCap) backoff <<= 1;
         if ((polls & 7) == 0 &&
-            (wall_clock64() - t0 > kWaitTicks ||
+            (wall_clock64() - t0 > g_wait_ticks ||
              __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
           atomicOr(err, 1);
           break;
@@ -309,7 +312,7 @@ __device__ __forceinline__ bool wave_resident(const unsigned long long *arrive,
       for (int i = 0; i < backoff; i++) __builtin_amdgcn_s_sleep(1);
       if (backoff < 8) backoff <<= 1;
       if ((polls & 15) == 0 &&
-          (wall_clock64() - t0 > kWaitTicks ||
+          (wall_clock64() - t0 > g_wait_ticks ||
            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
         atomicOr(err, 4);
         good = 0;

@@ -1828,6 +1828,15 @@ int rchol_amd_set_option(rchol_amd_ctx *c, int key, double value) {
     case RCHOL_AMD_OPT_STRICT_SYNC:
       c->strict_sync = v ? 1 : 0;
       break;
+    case RCHOL_AMD_OPT_WAIT_TICKS: {
+      if (!(value >= 0 && value <= 1e12))
+        return fail(RCHOL_AMD_EINVAL, "wait bound %g not in [0, 1e12] ticks", value);
+      const long long t = value == 0 ? kWaitTicks : (long long)value;
+      HIP_TRY(hipSetDevice(c->device));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_wait_ticks), &t, sizeof t));
+      break;
+    }
     case RCHOL_AMD_OPT_CHUNKED:
       if (v < -1 || v > 1) return fail(RCHOL_AMD_EINVAL, "chunked mode %d not in {-1, 0, 1}", v);
       if (c->hasG && v == 1 && c->fwd.ch.nchunks == 0)

@@ -63,7 +63,7 @@ __device__ __forceinline__ XT chunk_wait_value(const XT *x, int j, int *err) {
     v = Sent<XT>::load_agent(x + j);
     if (!Sent<XT>::is(v)) return v;
     if ((polls & 15) == 0 &&
-        (wall_clock64() - t0 > kWaitTicks ||
+        (wall_clock64() - t0 > g_wait_ticks ||
          __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
       atomicOr(err, 1);
       return XT(0);
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(kChunkThreads) void k_sptrsv_chunk(
         for (int i = 0; i < backoff; i++) __builtin_amdgcn_s_sleep(1);
         if (backoff < 16) backoff <<= 1;
         if ((polls & 15) == 0 &&
-            (wall_clock64() - t0 > kWaitTicks ||
+            (wall_clock64() - t0 > g_wait_ticks ||
              __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
           atomicOr(err, 1);
           break;

@@ -25,6 +25,7 @@ OPT_COHERENT_GATHER = 7
 OPT_TASK_WAVES = 8
 OPT_STRICT_SYNC = 9
 OPT_CHUNKED = 10
+OPT_WAIT_TICKS = 11
 
 
 class Stats(ctypes.Structure):
