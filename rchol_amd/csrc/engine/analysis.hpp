@@ -13,8 +13,8 @@
 //   2. the CSR of lower G in natural row order (a transposition through
 //      row buckets, with no atomic operations);
 //   3. levels of both sweeps over pieces of rows, each piece a task that
-//      runs once every piece it reads is done; then each row's entries
-//      sorted for its sweep;
+//      runs once every piece it reads is done; each row's entries sorted
+//      for its sweep (backward rows inside those tasks, forward rows after);
 //   4. slots (stable counting sort by level).
 // Rows stay in natural order, each sorted for its sweep in step 3; the
 // device gathers them into slot order.
@@ -523,9 +523,10 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
     });
   }
 
-  // 3. levels, then each row's entries sorted by level.  Forward: lev(i) =
-  // 1 + max lev(j) over row i (0 if empty), rows ascending; backward: lev(j)
-  // = 1 + max lev(i) over column j, columns descending.  Task q < P is
+  // 3. levels, and each row's entries sorted by level.  Forward: lev(i) =
+  // 1 + max lev(j) over row i (0 if empty), rows ascending, sorted in a pass
+  // of their own afterwards; backward: lev(j) = 1 + max lev(i) over column
+  // j, columns descending, each sorted as soon as its level is known.  Task q < P is
   // forward piece q, task P + q backward piece q; an entry from column
   // piece a to row piece b > a makes forward b wait for forward a and
   // backward a wait for backward b.
@@ -568,23 +569,37 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
         }
         topf[t] = std::max(topf[t], top);
       } else {
+        // a backward row (stored column) is final once its level is: it is
+        // sorted and stored here, where the pool has room beside the
+        // forward sweep's chain of separator pieces.  Read from its end:
+        // later rows finish earlier in this sweep.
+        std::vector<LevelEnt> buf;  // one per task: no cache line shared between threads
         for (size_t j = pstart[v - P + 1]; j-- > pstart[v - P];) {
+          buf.clear();
           int m = 0;
-          for (size_t e = colPtr[j]; e < colPtr[j + 1]; e++) {
+          for (size_t e = colPtr[j + 1]; e-- > colPtr[j];) {
             const size_t i = rowIdx[e];
-            if (i != j) m = std::max(m, levb[i] + 1);
+            if (i == j) continue;
+            const int l = levb[i];
+            m = std::max(m, l + 1);
+            buf.push_back({((unsigned long long)(unsigned)l << 32) | (unsigned)i, val[e]});
           }
           levb[j] = m;
           top = std::max(top, m);
+          const int k = (int)buf.size();
+          sort_level_ents(buf.data(), k);
+          const size_t o = colPtr[j] - j;
+          for (int u = 0; u < k; u++) {
+            bci[o + u] = (int)(unsigned)(buf[u].key & 0xffffffffu);
+            bcv[o + u] = buf[u].v;
+          }
         }
         topb[t] = std::max(topb[t], top);
       }
     });
-    // Every row's entries in sweep order, rows in parallel (chunks taken
-    // from a shared counter: the dense separator rows come last).  Forward
-    // rows are sorted in place, and most already are; backward rows (the
-    // stored columns) are read from their ends, since later rows finish
-    // earlier in that sweep, and written to the backward store.
+    // Forward rows in sweep order, in place, rows in parallel (chunks taken
+    // from a shared counter: the dense separator rows come last); most
+    // already are.
     std::atomic<size_t> next{0};
     constexpr size_t kRowChunk = 4096;
     par_run(T, [&](int) {
@@ -611,20 +626,6 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
           for (int u = 0; u < k; u++) {
             cin[e0 + u] = (int)(unsigned)(buf[u].key & 0xffffffffu);
             cvn[e0 + u] = buf[u].v;
-          }
-        }
-        for (size_t j = r0; j < r1; j++) {
-          buf.clear();
-          for (size_t e = colPtr[j + 1]; e-- > colPtr[j];) {
-            const size_t i = rowIdx[e];
-            if (i != j) buf.push_back({((unsigned long long)(unsigned)levb[i] << 32) | (unsigned)i, val[e]});
-          }
-          const int k = (int)buf.size();
-          sort_level_ents(buf.data(), k);
-          const size_t o = colPtr[j] - j;
-          for (int u = 0; u < k; u++) {
-            bci[o + u] = (int)(unsigned)(buf[u].key & 0xffffffffu);
-            bcv[o + u] = buf[u].v;
           }
         }
       }
