@@ -225,7 +225,7 @@ void run_dag(int V, const std::vector<int> &succ_ptr, const std::vector<int> &su
 // their nested-dissection order, so a run of slots is a compact piece of
 // the domain), then level-chunked 64-slot tasks.  Threads count their row
 // ranges per level when the per-thread counters are no larger than the rows.
-void build_slots(const IntVec &lev, int nlev, HostSweep &hs, IntVec &slot_of) {
+void build_slots(const IntVec &lev, int nlev, HostSweep &hs) {
   const int n = (int)lev.size();
   int T = (int)std::min<size_t>((size_t)host_threads(), std::max<size_t>(1, (size_t)n / (1 << 16)));
   if ((long long)T * (nlev + 1) > n) T = 1;
@@ -249,14 +249,10 @@ void build_slots(const IntVec &lev, int nlev, HostSweep &hs, IntVec &slot_of) {
     lp[nlev] = s;
   }
   hs.slot_row.resize(n);
-  slot_of.resize(n);
   par_run(T, [&](int t) {
     int *c = &cnt[(size_t)t * (nlev + 1)];
-    for (int i = (int)((long long)n * t / T); i < (int)((long long)n * (t + 1) / T); i++) {
-      const int s = c[lev[i]]++;
-      hs.slot_row[s] = i;
-      slot_of[i] = s;
-    }
+    for (int i = (int)((long long)n * t / T); i < (int)((long long)n * (t + 1) / T); i++)
+      hs.slot_row[c[lev[i]]++] = i;
   });
   hs.task_ptr.clear();
   for (int l = 0; l < nlev; l++) {
@@ -305,8 +301,9 @@ void sort_level_ents(LevelEnt *b, int k) {
 }
 
 // Host analysis of the stored factor: validation (lower triangle, one
-// nonzero diagonal per column), dependency levels of both sweeps and their
-// slot-ordered CSR.  Forward: row i of G waits for every column j < i with
+// nonzero diagonal per column), dependency levels of both sweeps, their
+// slots, and their rows in natural order with the entries in sweep order.
+// Forward: row i of G waits for every column j < i with
 // G(i,j) != 0.  Backward (G^T z = y): row i waits for every j > i with
 // G(j,i) != 0, i.e. the stored column i.
 struct StagedEnt {
@@ -635,9 +632,8 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
   nlb = *std::max_element(topb.begin(), topb.end()) + 1;
 
   // 4. slots, and each slot's entry range in the slot-order layout
-  IntVec slot_of_f, slot_of_b;
-  build_slots(levf, nlf, hf, slot_of_f);
-  build_slots(levb, nlb, hb, slot_of_b);
+  build_slots(levf, nlf, hf);
+  build_slots(levb, nlb, hb);
   for (HostSweep *h : {&hf, &hb}) {
     const IntVec &nr = h->nat_rp;
     const IntVec &sr = h->slot_row;
