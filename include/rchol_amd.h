@@ -112,6 +112,10 @@ typedef struct rchol_amd_schedule {
                         cuts narrow levels finer, by its resident waves) */
   int *level_ptr[2]; /* nlevels+1 slot offsets per level (malloc'ed) */
   int *slot_row[2];  /* n: row held by each slot, in level order (malloc'ed) */
+  unsigned short *proxy[2]; /* n: per slot, the distance from its row's end of
+                        the entry before the trailing entries of the level
+                        just below, when that entry is at most 3 levels
+                        below the row's; 0: none (malloc'ed) */
 } rchol_amd_schedule;
 
 /* Thread-local text of the last error on the calling thread. */

@@ -20,7 +20,8 @@ i32p = ctypes.POINTER(ctypes.c_int)
 class Schedule(ctypes.Structure):
     _fields_ = [("nlevels", ctypes.c_int * 2), ("head_end", ctypes.c_int * 2),
                 ("tail_begin", ctypes.c_int * 2), ("mid_tasks", ctypes.c_int * 2),
-                ("level_ptr", i32p * 2), ("slot_row", i32p * 2)]
+                ("level_ptr", i32p * 2), ("slot_row", i32p * 2),
+                ("proxy", ctypes.POINTER(ctypes.c_uint16) * 2)]
 
 
 # Every C-ABI symbol of include/rchol_amd.h with its signature.

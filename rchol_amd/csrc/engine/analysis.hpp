@@ -86,6 +86,7 @@ struct HostSweep {
   int *nat_ci = nullptr;
   double *nat_cv = nullptr;
   std::shared_ptr<const DblVec> nat_diag;  // pivots, natural order
+  std::vector<unsigned short, NoInit<unsigned short>> proxy;  // per slot (slot_proxies)
   int nlevels = 0;
 };
 
@@ -298,6 +299,52 @@ void sort_level_ents(LevelEnt *b, int k) {
       return;
     }
   }
+}
+
+// The proxy of a row, for the waits of the persistent sweep.  A row of
+// level m stores its k entries latest level last (level_at(u) of entry u
+// ascending), so its trailing entries are those of level m - 1; the proxy
+// is the distance from the row's end of the entry just before them, when
+// that entry lies at most RCHOL_AMD_PROXY_GAP levels below the row's own.
+// Once that value is published, the levels just below the row's are being
+// computed, and the wave can afford to poll each pending value itself.
+// 0: no such entry, a distance beyond 16 bits, or level 0 or 1.  Gap 3
+// measured best at 256^3 (forward sweep 1.58 / 1.55 / 1.59 ms for gap
+// 2 / 3 / any).
+#ifndef RCHOL_AMD_PROXY_GAP
+#define RCHOL_AMD_PROXY_GAP 3
+#endif
+using ProxyVec = std::vector<unsigned short, NoInit<unsigned short>>;
+
+template <typename LV>
+inline unsigned short row_proxy(int m, int k, LV &&level_at) {
+  if (m < 2) return 0;
+  int c = 0;
+  while (c < k && level_at(k - 1 - c) == m - 1) c++;
+  if (c >= k || c > 0xffff || level_at(k - 1 - c) < m - RCHOL_AMD_PROXY_GAP) return 0;
+  return (unsigned short)c;
+}
+
+// Per-slot proxies from per-row ones; 0 in levels of kProxyMaxRows rows or
+// more (those become stream tasks, which do not use proxies).
+constexpr int kProxyMaxRows = 1 << 18;
+
+void slot_proxies(size_t n, const ProxyVec &px_row, HostSweep &h) {
+  h.proxy.resize(n);
+  const std::vector<int> &lp = h.level_ptr;
+  par_for(n, [&](size_t s0, size_t s1) {
+    if (s0 >= s1) return;
+    int l = (int)(std::upper_bound(lp.begin(), lp.end(), (int)s0) - lp.begin()) - 1;
+    for (size_t s = s0; s < s1;) {
+      while ((size_t)lp[l + 1] <= s) l++;
+      const size_t e = std::min(s1, (size_t)lp[l + 1]);
+      if (lp[l + 1] - lp[l] >= kProxyMaxRows)
+        std::fill(h.proxy.begin() + s, h.proxy.begin() + e, (unsigned short)0);
+      else
+        for (size_t q = s; q < e; q++) h.proxy[q] = px_row[h.slot_row[q]];
+      s = e;
+    }
+  });
 }
 
 // Host analysis of the stored factor: validation (lower triangle, one
@@ -528,6 +575,7 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
   // piece a to row piece b > a makes forward b wait for forward a and
   // backward a wait for backward b.
   IntVec levf(n), levb(n);
+  ProxyVec pxf(n), pxb(n);  // per-row proxies (row_proxy)
   std::vector<int> topf(T, 0), topb(T, 0);
   {
     for (int t = 1; t < T; t++)
@@ -585,6 +633,7 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
           top = std::max(top, m);
           const int k = (int)buf.size();
           sort_level_ents(buf.data(), k);
+          pxb[j] = row_proxy(m, k, [&](int u) { return (int)(buf[u].key >> 32); });
           const size_t o = colPtr[j] - j;
           for (int u = 0; u < k; u++) {
             bci[o + u] = (int)(unsigned)(buf[u].key & 0xffffffffu);
@@ -613,17 +662,19 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
             sorted = u == 0 || key > prev;
             prev = key;
           }
-          if (sorted) continue;
-          buf.resize(k);
-          for (int u = 0; u < k; u++) {
-            const int j = cin[e0 + u];
-            buf[u] = {((unsigned long long)(unsigned)levf[j] << 32) | (unsigned)j, cvn[e0 + u]};
+          if (!sorted) {
+            buf.resize(k);
+            for (int u = 0; u < k; u++) {
+              const int j = cin[e0 + u];
+              buf[u] = {((unsigned long long)(unsigned)levf[j] << 32) | (unsigned)j, cvn[e0 + u]};
+            }
+            sort_level_ents(buf.data(), k);
+            for (int u = 0; u < k; u++) {
+              cin[e0 + u] = (int)(unsigned)(buf[u].key & 0xffffffffu);
+              cvn[e0 + u] = buf[u].v;
+            }
           }
-          sort_level_ents(buf.data(), k);
-          for (int u = 0; u < k; u++) {
-            cin[e0 + u] = (int)(unsigned)(buf[u].key & 0xffffffffu);
-            cvn[e0 + u] = buf[u].v;
-          }
+          pxf[i] = row_proxy(levf[i], k, [&](int u) { return levf[cin[e0 + u]]; });
         }
       }
     });
@@ -631,7 +682,7 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
   nlf = *std::max_element(topf.begin(), topf.end()) + 1;
   nlb = *std::max_element(topb.begin(), topb.end()) + 1;
 
-  // 4. slots, and each slot's entry range in the slot-order layout
+  // 4. slots, each slot's entry range in the slot-order layout, proxies
   build_slots(levf, nlf, hf);
   build_slots(levb, nlb, hb);
   for (HostSweep *h : {&hf, &hb}) {
@@ -639,6 +690,7 @@ static int analyze_factor(size_t n, const size_t *colPtr, const size_t *rowIdx, 
     const IntVec &sr = h->slot_row;
     par_prefix(n, [&](size_t s) { return nr[sr[s] + 1] - nr[sr[s]]; }, h->rp);
     h->nat_diag = diag_p;
+    slot_proxies(n, h == &hf ? pxf : pxb, *h);
   }
   return RCHOL_AMD_OK;
 }

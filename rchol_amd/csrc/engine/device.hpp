@@ -169,6 +169,80 @@ __device__ __forceinline__ void await_batch(const T *x, const int (&j)[U], T (&x
   }
 }
 
+// Back-off cap of the all-lane phase of await_proxy, in s_sleep(1) units.
+#ifndef RCHOL_AMD_POLL2_CAP
+#define RCHOL_AMD_POLL2_CAP 1
+#endif
+constexpr int kPoll2Cap = RCHOL_AMD_POLL2_CAP;
+
+// await_batch for a round whose wave knows a proxy: slot pu of lane pl
+// holds an entry a level or two below the latest of the round (the host's
+// choice), so once it is published the latest levels are being computed.  Phase 1: lane pl polls the proxy with back-off (one address,
+// however long the wait).  Phase 2: every lane re-reads its own pending
+// slots each poll, so a value is seen one load after its publication
+// instead of after a poll and a second, re-reading load.  The entries
+// still pending in phase 2 are few (most of a row's entries lie below the
+// latest level).  Same bound, error flag and zeroing as await_batch.
+template <typename T, int U>
+__device__ __forceinline__ void await_proxy(const T *x, const int (&j)[U], T (&xv)[U], int *err,
+                                            int pl, int pu) {
+  bool pend = false;
+#pragma unroll
+  for (int u = 0; u < U; u++)
+    if (j[u] >= 0 && Sent<T>::is(xv[u])) xv[u] = Sent<T>::load_agent(x + j[u]);
+#pragma unroll
+  for (int u = 0; u < U; u++) pend |= j[u] >= 0 && Sent<T>::is(xv[u]);
+  unsigned long long mask = __ballot(pend);
+  if (!mask) return;
+  const int lane = __lane_id();
+  const long long t0 = wall_clock64();
+  int jp = -1, wait = 0;
+#pragma unroll
+  for (int u = 0; u < U; u++)
+    if (u == pu) {
+      jp = j[u];
+      wait = Sent<T>::is(xv[u]) ? 1 : 0;
+    }
+  jp = __shfl(jp, pl, 64);
+  wait = __shfl(wait, pl, 64);
+  if (wait && lane == pl) {
+    int backoff = 1;
+    for (int polls = 1;; polls++) {
+      for (int i = 0; i < backoff; i++) __builtin_amdgcn_s_sleep(1);
+      if (!Sent<T>::is(Sent<T>::load_agent(x + jp))) break;
+      if (backoff < kPollCap) backoff <<= 1;
+      if ((polls & 7) == 0 &&
+          (wall_clock64() - t0 > g_wait_ticks ||
+           __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+        atomicOr(err, 1);
+        break;
+      }
+    }
+  }
+  int backoff = 1;
+  for (int polls = 1;; polls++) {
+    pend = false;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (j[u] >= 0 && Sent<T>::is(xv[u])) xv[u] = Sent<T>::load_agent(x + j[u]);
+#pragma unroll
+    for (int u = 0; u < U; u++) pend |= j[u] >= 0 && Sent<T>::is(xv[u]);
+    mask = __ballot(pend);
+    if (!mask) return;
+    if ((polls & 7) == 0 &&
+        (wall_clock64() - t0 > g_wait_ticks ||
+         __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+      if (lane == __ffsll((long long)mask) - 1) atomicOr(err, 1);
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (j[u] >= 0 && Sent<T>::is(xv[u])) xv[u] = T(0);
+      return;
+    }
+    for (int i = 0; i < backoff; i++) __builtin_amdgcn_s_sleep(1);
+    if (backoff < kPoll2Cap) backoff <<= 1;
+  }
+}
+
 // Sum over the L consecutive lanes of a row group (L a power of two <= 64).
 template <int L>
 __device__ __forceinline__ double group_sum(double v) {

@@ -132,6 +132,7 @@ struct Sweep {
   // split schedule: levels [0, head_end) and [tail_begin, nlevels) run in one
   // workgroup (k_sptrsv_serial), the middle in the persistent kernel
   std::vector<int> level_ptr_h, rp_h;
+  std::vector<unsigned short> proxy_h;  // per slot: proxy distance (0: none)
   DevBuf<int> level_ptr, level_lg;
   DevBuf<int> wave_ptr, wt_s0, wt_info;  // middle tasks in per-wave lists
   int head_end = 0, tail_begin = 0, mid_ntasks = 0;
@@ -212,14 +213,26 @@ struct TaskCut {
   std::vector<int> s0, info, level;  // info = cnt | lg << kTaskInfoShift
 };
 
-void cut_tasks(const std::vector<int> &lp, const std::vector<int> &rp, int h, int t, int lmin_lg,
+void cut_tasks(const std::vector<int> &lp, const std::vector<int> &rp,
+               const std::vector<unsigned short> &proxy, int h, int t, int lmin_lg,
                long long nwaves, bool flat_ok, TaskCut &tc) {
   tc.s0.clear();
   tc.info.clear();
   tc.level.clear();
   auto push = [&](int s, int cnt, int lg, int l) {
+    // a lane task's proxy: that of its first row that has one in the first
+    // round (one pass: row k is row group k)
+    int px = 0;
+    if (RCHOL_AMD_PROXY && lg < kTaskStream && (size_t)(s + cnt) <= proxy.size())
+      for (int k = 0; k < cnt; k++) {
+        const int pc = proxy[s + k];
+        if (pc > 0 && pc < (kGather << lg)) {
+          px = pc | (k << 9);
+          break;
+        }
+      }
     tc.s0.push_back(s);
-    tc.info.push_back(cnt | (lg << kTaskInfoShift));
+    tc.info.push_back(cnt | (lg << kTaskInfoShift) | (px << kTaskProxyShift));
     tc.level.push_back(l);
   };
   for (int l = h; l < t; l++) {
@@ -685,7 +698,8 @@ int rchol_amd_ctx::size_trsv_grid() {
   size_t max_tasks = 0;
   for (int k = 0; k < 2; k++) {
     compute_split(sw[k]->level_ptr_h, sw[k]->nlevels, serial_width, h[k], t[k]);
-    cut_tasks(sw[k]->level_ptr_h, sw[k]->rp_h, h[k], t[k], lmin_lg, plan_waves, k == 0, tc[k]);
+    cut_tasks(sw[k]->level_ptr_h, sw[k]->rp_h, sw[k]->proxy_h, h[k], t[k], lmin_lg, plan_waves,
+              k == 0, tc[k]);
     max_tasks = std::max(max_tasks, tc[k].s0.size());
   }
   trsv_grid = (int)std::max<size_t>(1, std::min<size_t>(cap_blocks, (max_tasks + 3) / 4));
@@ -1754,6 +1768,7 @@ int rchol_amd_upload_G(rchol_amd_ctx *c, size_t n, const size_t *colPtr, const s
     HIP_TRY(sw.task_ptr.upload(h.task_ptr.data(), h.task_ptr.size()));
     sw.level_ptr_h = h.level_ptr;
     sw.rp_h.assign(h.rp.begin(), h.rp.end());
+    sw.proxy_h.assign(h.proxy.begin(), h.proxy.end());
     HIP_TRY(sw.level_ptr.upload(h.level_ptr.data(), h.level_ptr.size()));
     HIP_TRY(sw.slot_row.upload(h.slot_row.data(), n));
     HIP_TRY(sw.rp.upload(h.rp.data(), n + 1));
@@ -2334,9 +2349,12 @@ int rchol_amd_analyze_G(size_t n, const size_t *colPtr, const size_t *rowIdx, co
     out->mid_tasks[k] = tasks;
     out->level_ptr[k] = (int *)malloc((nl[k] + 1) * sizeof(int));
     out->slot_row[k] = (int *)malloc(n * sizeof(int));
-    if (!out->level_ptr[k] || !out->slot_row[k]) return fail(RCHOL_AMD_EINVAL, "out of memory");
+    out->proxy[k] = (unsigned short *)malloc(n * sizeof(unsigned short));
+    if (!out->level_ptr[k] || !out->slot_row[k] || !out->proxy[k])
+      return fail(RCHOL_AMD_EINVAL, "out of memory");
     memcpy(out->level_ptr[k], h[k].level_ptr.data(), (nl[k] + 1) * sizeof(int));
     memcpy(out->slot_row[k], h[k].slot_row.data(), n * sizeof(int));
+    memcpy(out->proxy[k], h[k].proxy.data(), n * sizeof(unsigned short));
   }
   return RCHOL_AMD_OK;
 }
@@ -2506,8 +2524,10 @@ void rchol_amd_schedule_free(rchol_amd_schedule *s) {
   for (int k = 0; k < 2; k++) {
     free(s->level_ptr[k]);
     free(s->slot_row[k]);
+    free(s->proxy[k]);
     s->level_ptr[k] = nullptr;
     s->slot_row[k] = nullptr;
+    s->proxy[k] = nullptr;
   }
 }
 

@@ -44,16 +44,24 @@ namespace rchol_amd {
 #define RCHOL_AMD_GATHER 8
 #endif
 constexpr int kGather = RCHOL_AMD_GATHER;  // gathers in flight per lane and round
+#ifndef RCHOL_AMD_PROXY
+#define RCHOL_AMD_PROXY 1
+#endif
 
 // Sum of val[e] * x[ci[e]] over the entries e1 - 1 - sub - k L >= e0 of one
 // row, waiting for unpublished x.  The row is read from its end: its entries
 // are stored latest dependency last, so slot 0 of the first round -- the one
 // await_batch polls -- holds the value produced last.  Wave-converged call
 // (every lane of the wave, e0 == e1 for idle groups).
+//
+// px = pc | g << 9 with pc > 0: the first round holds a proxy
+// (await_proxy) at distance pc from the end of the row of row group g, i.e.
+// in lane g L + pc % L, slot pc / L; the task cut only sets it when
+// pc < kGather * L.
 template <typename VT, typename XT, int L, bool WAIT>
 __device__ __forceinline__ XT row_dot(int e0, int e1, int sub, const int *__restrict__ ci,
                                       const VT *__restrict__ cv, const XT *x, int *err,
-                                      int coherent) {
+                                      int coherent, int px = 0) {
   XT acc = 0;
   for (int c = e1 - 1 - sub; c >= e0; c -= kGather * L) {
     int j[kGather];
@@ -78,7 +86,14 @@ __device__ __forceinline__ XT row_dot(int e0, int e1, int sub, const int *__rest
       bool pend = false;
 #pragma unroll
       for (int u = 0; u < kGather; u++) pend |= Sent<XT>::is(xv[u]);
-      if (__ballot(pend)) await_batch<XT, kGather>(x, j, xv, err);
+      if (__ballot(pend)) {
+        const int pc = px & 511;
+        if (RCHOL_AMD_PROXY && pc > 0)
+          await_proxy<XT, kGather>(x, j, xv, err, (px >> 9) * L + pc % L, pc / L);
+        else
+          await_batch<XT, kGather>(x, j, xv, err);
+      }
+      px = 0;  // later rounds: entries of lower levels
     }
 #pragma unroll
     for (int u = 0; u < kGather; u++) acc += (XT)v[u] * xv[u];
@@ -93,7 +108,7 @@ __device__ __forceinline__ void sweep_task(int cnt, int m_row, int m_e0, int m_e
                                            const int *__restrict__ ci, const VT *__restrict__ cv,
                                            const BT *__restrict__ rhs, XT *x,
                                            XT *__restrict__ aux, const double *__restrict__ dotv,
-                                           int *err, int coherent, double &dacc) {
+                                           int *err, int coherent, double &dacc, int pc) {
   constexpr int R = 64 / L;
   const int lane = threadIdx.x & 63;
   const int grp = lane / L, sub = lane % L;
@@ -105,7 +120,7 @@ __device__ __forceinline__ void sweep_task(int cnt, int m_row, int m_e0, int m_e
     const bool live = k < cnt;
     if (!live) e1 = e0;
     const BT b = (live && sub == 0) ? ld_stream(rhs + row) : (BT)0;
-    XT acc = row_dot<VT, XT, L, true>(e0, e1, sub, ci, cv, x, err, coherent);
+    XT acc = row_dot<VT, XT, L, true>(e0, e1, sub, ci, cv, x, err, coherent, base == 0 ? pc : 0);
     acc = group_sum<L>(acc);
     if (live && sub == 0) {
       const XT v = ((XT)b - acc) / (XT)dg;
@@ -131,6 +146,7 @@ constexpr int kTaskStream = 7;             // lane code of a stream task
 constexpr int kTaskFlat = 8;               // lane code of a flat task (see sweep_flat)
 constexpr int kFlatPerLane = 8;            // slots per lane of a flat task
 constexpr int kTaskInfoShift = 10;         // task info word: slot count | lane code << 10
+constexpr int kTaskProxyShift = 14;        //   | proxy (lane tasks: distance | row group << 9) << 14
 
 // acc + buf[lo] + buf[lo + 1] + ... + buf[hi - 1], added in that order, with
 // four LDS reads in flight at a time (a plain loop waits for each read).
@@ -267,7 +283,7 @@ __global__ __launch_bounds__(256) void k_sptrsv(
   auto fetch = [&](int kk) {
     n_s0 = wt_s0[kk];
     n_info = wt_info[kk];
-    if ((n_info >> kTaskInfoShift) != kTaskFlat &&
+    if (((n_info >> kTaskInfoShift) & 15) != kTaskFlat &&
         lane < (n_info & ((1 << kTaskInfoShift) - 1))) {
       n_row = ld_stream(slot_row + n_s0 + lane);
       n_e0 = ld_stream(rp + n_s0 + lane);
@@ -278,8 +294,9 @@ __global__ __launch_bounds__(256) void k_sptrsv(
   if (k < k1) fetch(k);
   for (; k < k1; k++) {
     const int info = n_info, m_s0 = n_s0;
-    const int lg = info >> kTaskInfoShift;
+    const int lg = (info >> kTaskInfoShift) & 15;
     const int cnt = info & ((1 << kTaskInfoShift) - 1);  // <= 64 slots (flat: 64 kFlatPerLane)
+    const int pc = info >> kTaskProxyShift;
     const int m_row = n_row, m_e0 = n_e0, m_e1 = n_e1;
     const VT m_dg = n_dg;
     if (!checked && lg != kTaskFlat) {  // the first task that may wait
@@ -289,7 +306,7 @@ __global__ __launch_bounds__(256) void k_sptrsv(
     if (k + 1 < k1) fetch(k + 1);
 #define RCHOL_SWEEP_TASK(LL)                                                                  \
   sweep_task<VT, XT, BT, LL, FUSE>(cnt, m_row, m_e0, m_e1, m_dg, ci, cv, rhs, x, aux, dotv, err, \
-                                   coherent, dacc)
+                                   coherent, dacc, pc)
     switch (lg) {
       case kTaskFlat:  // forward sweeps only (FUSE 0 / 1): keeps the r.z kernel's registers
         if constexpr (FUSE != 2)

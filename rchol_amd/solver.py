@@ -343,7 +343,8 @@ def stack_systems(systems):
 
 def analyze(G, serial_width=0):
     """Host-only level schedule of the two sweeps for a factor (stored triple
-    or scipy lower G): dict per sweep with levels, split and slot order."""
+    or scipy lower G): dict per sweep with levels, split, slot order and the
+    per-slot proxies of the persistent sweep's waits."""
     if isinstance(G, tuple):
         cp, ri, gv = G
     else:
@@ -363,6 +364,7 @@ def analyze(G, serial_width=0):
             "mid_tasks": sc.mid_tasks[k],
             "level_ptr": np.ctypeslib.as_array(sc.level_ptr[k], shape=(nl + 1,)).copy(),
             "slot_row": np.ctypeslib.as_array(sc.slot_row[k], shape=(n,)).copy(),
+            "proxy": np.ctypeslib.as_array(sc.proxy[k], shape=(n,)).copy(),
         }
     lib.rchol_amd_schedule_free(ctypes.byref(sc))
     return out

@@ -7,7 +7,9 @@ no code or order of work with the threaded piece scheduler it checks.
 Shapes cover one row, no off-diagonal entries, a star (one row read by all,
 reading all), a chain, rows alternating between reading and not reading
 (more nested-dissection block starts than the scheduler keeps), and
-factors large enough to split over host threads."""
+factors large enough to split over host threads.  The per-slot proxies of
+the sweep's waits are restated from the same levels, without the entry
+order they are read from."""
 import numpy as np
 import pytest
 
@@ -28,8 +30,22 @@ def tight_levels(cp, ri, n):
             if np.array_equal(new, lev):
                 break
             lev = new
-        out[key] = lev
+        out[key] = (lev, dst, src)
     return out
+
+
+def row_proxies(lev, dst, src):
+    """Per row of level l >= 2: c = its dependencies of level l - 1 (a row
+    stores them last), kept when its deepest other dependency is at least
+    level l - 3 (the row's entry just before the trailing ones), else 0."""
+    n = lev.size
+    dl, rl = lev[src], lev[dst]
+    c = np.bincount(dst[dl == rl - 1], minlength=n)
+    below = np.full(n, -1, dtype=np.int64)
+    m = dl < rl - 1
+    np.maximum.at(below, dst[m], dl[m])
+    ok = (lev >= 2) & (below >= 0) & (below >= lev - 3) & (c <= 0xFFFF)
+    return np.where(ok, c, 0)
 
 
 def check(Gs):
@@ -38,12 +54,16 @@ def check(Gs):
     sch = rchol_amd.analyze((cp.astype(np.uint64), ri.astype(np.uint64), gv.astype(np.float64)), 0)
     want = tight_levels(cp, ri, n)
     for key in ("fwd", "bwd"):
-        lev = want[key]
+        lev, dst, src = want[key]
         s = sch[key]
         assert s["nlevels"] == int(lev.max()) + 1
         np.testing.assert_array_equal(s["level_ptr"],
                                       np.concatenate([[0], np.cumsum(np.bincount(lev))]))
         np.testing.assert_array_equal(s["slot_row"], np.lexsort((np.arange(n), lev)))
+        # proxies: per slot, those of rows in levels of fewer than 2^18 rows
+        width = np.bincount(lev)[lev]
+        px = np.where(width < (1 << 18), row_proxies(lev, dst, src), 0)
+        np.testing.assert_array_equal(s["proxy"], px[s["slot_row"]])
 
 
 def lower(n, rows, cols, seed=0):

@@ -4,7 +4,9 @@ must wait reports RCHOL_AMD_ESTALL instead of hanging; every kernel still
 exits, and once the bound is restored the same context solves correctly
 (checked against the oracle).  Both sweep forms: level by level and in
 chunks.  The factor is a 1D chain in natural order, so every row waits for
-the one before it."""
+the one before it; in the band case every row also reads the row two
+before it, so the level-by-level waits go through a proxy (an entry two
+levels down) before every lane polls its own pending values."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -22,22 +24,25 @@ def need_gpu():
         pytest.skip("no HIP device")
 
 
-def chain(n, seed):
-    w = np.random.default_rng(seed).uniform(0.5, 2.0, n - 1)
-    A = sp.diags([-w, -w], [-1, 1], shape=(n, n), format="csr")
-    d = np.zeros(n)
-    d[:-1] += w
-    d[1:] += w
-    A = (A + sp.diags(d + 1e-3)).tocsr()
+def chain(n, seed, band=1):
+    rng = np.random.default_rng(seed)
+    A = sp.csr_matrix((n, n))
+    d = np.zeros(n) + 1e-3
+    for k in range(1, band + 1):
+        w = rng.uniform(0.5, 2.0, n - k)
+        A = A + sp.diags([-w, -w], [-k, k], shape=(n, n))
+        d[:-k] += w
+        d[k:] += w
+    A = (A + sp.diags(d)).tocsr()
     A.sort_indices()
     Gs, _, _ = F.factor(A, None, [n], "static", seed)
     return A, Gs
 
 
-@pytest.mark.parametrize("chunked,n", [(0, 20000), (1, 70000)])
-def test_lowered_wait_bound_reports_a_stall_then_recovers(chunked, n):
-    print(f"case stall chunked={chunked} n={n}", flush=True)
-    A, Gs = chain(n, 3 + chunked)
+@pytest.mark.parametrize("chunked,n,band", [(0, 20000, 1), (1, 70000, 1), (0, 20000, 2)])
+def test_lowered_wait_bound_reports_a_stall_then_recovers(chunked, n, band):
+    print(f"case stall chunked={chunked} n={n} band={band}", flush=True)
+    A, Gs = chain(n, 3 + chunked, band)
     b = np.random.default_rng(9).uniform(-1, 1, n)
     xo, _, io = O.pcg(A, b, 1e-10, 50, Gs)
     with rchol_amd.Solver(A, Gs, rchol_amd.FP64, chunked=chunked) as s:
