@@ -115,11 +115,16 @@ def test_c3_pcg_iterations_match_oracle(c3):
         x, relres, itr = s.pcg(b, TOL, 2000)
     with rchol_amd.Solver(A, Gs, rchol_amd.FP32) as s:
         x32, relres32, itr32 = s.pcg(b, TOL, 2000)
+        # the fenced reduction handshake gives the same bits at this size too
+        s.set_option(rchol_amd.solver.OPT_STRICT_SYNC, 1)
+        x32s, relres32s, itr32s = s.pcg(b, TOL, 2000)
     print(f"C3 itr fp64 {itr} mixed {itr32} oracle {io}; x {rel(x, xo):.1e}", flush=True)
     assert abs(itr - io) <= 1
     assert rel(x, xo) <= X_RTOL
     assert true_relres(A, x, b) <= TOL * 10
     assert true_relres(A, x32, b) <= TOL * 10
+    np.testing.assert_array_equal(x32s, x32)
+    assert (relres32s, itr32s) == (relres32, itr32)
 
 
 def test_c4a_batch_of_32_against_oracle(c3):
