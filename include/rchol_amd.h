@@ -115,7 +115,8 @@ typedef struct rchol_amd_schedule {
   unsigned short *proxy[2]; /* n: per slot, the distance from its row's end of
                         the entry before the trailing entries of the level
                         just below, when that entry is at most 3 levels
-                        below the row's; 0: none (malloc'ed) */
+                        below the row's; 0: none, and in levels of 2^18
+                        rows or more (malloc'ed) */
 } rchol_amd_schedule;
 
 /* Thread-local text of the last error on the calling thread. */
