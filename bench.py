@@ -38,7 +38,8 @@ def kernel_bytes(n, nnz_a, nnz_g, fp32):
     sx = 4 if fp32 else 8   # sweep vector bytes (y, z)
     fwd = nnz_g * (sv + 4) + 4 * (n + 1) + 8 * n + sx * n
     bwd = nnz_g * (sv + 4) + 4 * (n + 1) + sx * n + sx * n + 8 * n
-    spmv = 12 * nnz_a + 4 * (n + 1) + sx * n + 8 * n + 8 * n + 16 * n
+    # p update (z, p read, p' written) + SpMV (A, p' and r read, q written)
+    spmv = 12 * nnz_a + 4 * (n + 1) + sx * n + 8 * n + 8 * n + 16 * n + 8 * n
     upd = 32 * n + 16 * n + sx * n
     return [fwd, bwd, spmv, upd]
 

@@ -605,8 +605,10 @@ template <typename ZT>
 void launch_spmv_p(rchol_amd_ctx *c, const ZT *z, const double *p_old, double *p_new) {
   switch (c->spmv_lanes) {
     case 0:
-      hipLaunchKernelGGL((k_spmv_p_s<ZT>), dim3(c->spmv_grid()), dim3(kBlock), 0, c->stream,
-                         (int)c->n, c->arp.p, c->aci.p, c->av.p, z, p_old, p_new, c->q.p, c->r.p,
+      hipLaunchKernelGGL((k_p_update<ZT>), dim3(c->vec_grid()), dim3(kBlock), 0, c->stream,
+                         (int)c->n, z, p_old, p_new, c->sc);
+      hipLaunchKernelGGL(k_spmv_p_s, dim3(c->spmv_grid()), dim3(kBlock), 0, c->stream, (int)c->n,
+                         c->arp.p, c->aci.p, c->av.p, (const double *)p_new, c->q.p, c->r.p,
                          c->partials.p, c->sc);
       break;
     case 4: launch_spmv_p_t<ZT, 4>(c, z, p_old, p_new); break;

@@ -724,38 +724,46 @@ __device__ __forceinline__ double stream_rows(int r0, int n, const int *__restri
   return acc;
 }
 
-// k_spmv_p in stream form (same results up to the order of each row sum).
+// The p update of the stream form, p' = z + beta p (p' = z at k == 0,
+// pcg.cpp:86-96), as a streaming pass of its own: the SpMV after it then
+// gathers one vector (p') instead of recomputing each neighbour's p' from
+// two gathers (z and p), which halves the scattered reads that bound it.
 template <typename ZT>
+__global__ __launch_bounds__(256) void k_p_update(int n, const ZT *__restrict__ z,
+                                                  const double *__restrict__ p_old,
+                                                  double *__restrict__ p_new,
+                                                  const DevScalars *sc) {
+  if (sc->done) return;
+  const double beta = sc->beta;
+  const bool first = (sc->k == 0);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p_new[i] = first ? (double)z[i] : beta * p_old[i] + (double)z[i];
+}
+
+// k_spmv_p in stream form, after k_p_update: q = A p', p'.r, p'.q and alpha
+// (same results up to the order of each row sum).
 __global__ __launch_bounds__(256) void k_spmv_p_s(int n, const int *__restrict__ rp,
                                                   const int *__restrict__ ci,
                                                   const double *__restrict__ av,
-                                                  const ZT *__restrict__ z,
-                                                  const double *__restrict__ p_old,
-                                                  double *__restrict__ p_new,
+                                                  const double *__restrict__ p,
                                                   double *__restrict__ q,
                                                   const double *__restrict__ r, double *partials,
                                                   DevScalars *sc) {
   if (sc->done) return;
   __shared__ double sbuf[4][64 * kSpmvStream];
   double *buf = sbuf[(threadIdx.x >> 6) & 3];
-  const double beta = sc->beta;
-  const bool first = (sc->k == 0);
   const int lane = threadIdx.x & 63;
   const RowWalk rw = xcd_rows(n);
-  auto pval = [&](int j) -> double {
-    return first ? (double)z[j] : beta * p_old[j] + (double)z[j];
-  };
   double apr = 0.0, apq = 0.0;
   for (int r0 = rw.first; r0 < rw.hi; r0 += rw.step) {
     const int row = r0 + lane;
     double pi = 0.0, ri = 0.0;
     if (row < rw.hi) {
-      pi = pval(row);
+      pi = p[row];
       ri = r[row];
     }
-    const double acc = stream_rows(r0, rw.hi, rp, ci, av, pval, buf);
+    const double acc = stream_rows(r0, rw.hi, rp, ci, av, [&](int j) { return p[j]; }, buf);
     if (row < rw.hi) {
-      p_new[row] = pi;
       q[row] = acc;
       apr += pi * ri;
       apq += pi * acc;

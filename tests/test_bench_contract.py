@@ -26,5 +26,5 @@ def test_result_line_fields():
     b64 = bench.kernel_bytes(100, 700, 400, False)
     b32 = bench.kernel_bytes(100, 700, 400, True)
     assert b64[0] - b32[0] == 4 * 400 + 4 * 100      # values and the y vector
-    assert b64[2] - b32[2] == 4 * 100                # z read by the SpMV
+    assert b64[2] - b32[2] == 4 * 100                # z read by the p update
     assert all(x > 0 for x in b64 + b32)
