@@ -1218,15 +1218,16 @@ namespace {
 template <int L>
 void launch_spmv_p_seg_t(rchol_amd_ctx *c, SegLayout lay, const void *z, bool z32,
                          const double *p_old, double *p_new) {
-  if (L == 0) {  // stream form
+  if (L == 0) {  // stream form: p update, then the SpMV gathering p'
     if (z32)
-      hipLaunchKernelGGL((k_spmv_p_seg_s<float>), dim3(c->segs.nblocks), dim3(kBlock), 0,
-                         c->stream, lay, c->arp.p, c->aci.p, c->av.p, (const float *)z, p_old,
-                         p_new, c->q.p, (const double *)c->r.p, c->partials.p, c->segs.ss);
+      hipLaunchKernelGGL((k_p_update_seg<float>), dim3(c->segs.nblocks), dim3(kBlock), 0,
+                         c->stream, lay, (const float *)z, p_old, p_new, c->segs.ss);
     else
-      hipLaunchKernelGGL((k_spmv_p_seg_s<double>), dim3(c->segs.nblocks), dim3(kBlock), 0,
-                         c->stream, lay, c->arp.p, c->aci.p, c->av.p, (const double *)z, p_old,
-                         p_new, c->q.p, (const double *)c->r.p, c->partials.p, c->segs.ss);
+      hipLaunchKernelGGL((k_p_update_seg<double>), dim3(c->segs.nblocks), dim3(kBlock), 0,
+                         c->stream, lay, (const double *)z, p_old, p_new, c->segs.ss);
+    hipLaunchKernelGGL(k_spmv_p_seg_s, dim3(c->segs.nblocks), dim3(kBlock), 0, c->stream, lay,
+                       c->arp.p, c->aci.p, c->av.p, (const double *)p_new, c->q.p,
+                       (const double *)c->r.p, c->partials.p, c->segs.ss);
     return;
   }
   constexpr int LL = L ? L : 8;

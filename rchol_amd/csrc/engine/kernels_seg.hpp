@@ -191,17 +191,32 @@ __global__ __launch_bounds__(256) void k_spmv_p_seg(SegLayout lay, const int *__
   }
 }
 
-// k_spmv_p_seg in stream form (RCHOL_AMD_OPT_SPMV_LANES = 0): the waves of
-// a block walk its row range 64 rows at a time (stream_rows).
+// The p update of the stream form, per system (p' = z + beta_s p; p' = z at
+// its k == 0, pcg.cpp:86-96), as a streaming pass of its own over the same
+// blocks: the SpMV after it gathers p' only, one scattered read per entry
+// instead of two.
+template <typename ZT>
+__global__ __launch_bounds__(256) void k_p_update_seg(SegLayout lay, const ZT *__restrict__ z,
+                                                      const double *__restrict__ p_old,
+                                                      double *__restrict__ p_new,
+                                                      const SegScalars *ss) {
+  if (ss->all_done) return;
+  const int s = lay.seg[blockIdx.x];
+  const bool first = ss->k[s] == 0;
+  const double beta = ss->beta[s];
+  for (int i = lay.r0[blockIdx.x] + threadIdx.x; i < lay.r1[blockIdx.x]; i += blockDim.x)
+    p_new[i] = first ? (double)z[i] : beta * p_old[i] + (double)z[i];
+}
+
+// k_spmv_p_seg in stream form (RCHOL_AMD_OPT_SPMV_LANES = 0), after
+// k_p_update_seg: the waves of a block walk its row range 64 rows at a time
+// (stream_rows), gathering p'.
 // Occupancy 8 (<= 64 VGPRs): the grid fills 8 workgroups per CU, and at 7
 // resident the eighth would run as a second wave of workgroups.
-template <typename ZT>
 __global__ __launch_bounds__(256, 8) void k_spmv_p_seg_s(SegLayout lay, const int *__restrict__ rp,
                                                       const int *__restrict__ ci,
                                                       const double *__restrict__ av,
-                                                      const ZT *__restrict__ z,
-                                                      const double *__restrict__ p_old,
-                                                      double *__restrict__ p_new,
+                                                      const double *__restrict__ p,
                                                       double *__restrict__ q,
                                                       const double *__restrict__ r,
                                                       double *partials, SegScalars *ss) {
@@ -209,11 +224,6 @@ __global__ __launch_bounds__(256, 8) void k_spmv_p_seg_s(SegLayout lay, const in
   __shared__ double sbuf[4][64 * kSpmvStream];
   double *buf = sbuf[(threadIdx.x >> 6) & 3];
   const int s = lay.seg[blockIdx.x];
-  const bool first = ss->k[s] == 0;
-  const double beta = ss->beta[s];
-  auto pval = [&](int j) -> double {
-    return first ? (double)z[j] : beta * p_old[j] + (double)z[j];
-  };
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
   const int hi = lay.r1[blockIdx.x];
   double apr = 0.0, apq = 0.0;
@@ -221,12 +231,11 @@ __global__ __launch_bounds__(256, 8) void k_spmv_p_seg_s(SegLayout lay, const in
     const int row = r0 + lane;
     double pi = 0.0, ri = 0.0;
     if (row < hi) {
-      pi = pval(row);
+      pi = p[row];
       ri = r[row];
     }
-    const double acc = stream_rows(r0, hi, rp, ci, av, pval, buf);
+    const double acc = stream_rows(r0, hi, rp, ci, av, [&](int j) { return p[j]; }, buf);
     if (row < hi) {
-      p_new[row] = pi;
       q[row] = acc;
       apr += pi * ri;
       apq += pi * acc;
