@@ -108,7 +108,7 @@ __device__ __forceinline__ void sweep_task(int cnt, int m_row, int m_e0, int m_e
                                            const int *__restrict__ ci, const VT *__restrict__ cv,
                                            const BT *__restrict__ rhs, XT *x,
                                            XT *__restrict__ aux, const double *__restrict__ dotv,
-                                           int *err, int coherent, double &dacc, int pc) {
+                                           int *err, int coherent, double &dacc, int px) {
   constexpr int R = 64 / L;
   const int lane = threadIdx.x & 63;
   const int grp = lane / L, sub = lane % L;
@@ -120,7 +120,7 @@ __device__ __forceinline__ void sweep_task(int cnt, int m_row, int m_e0, int m_e
     const bool live = k < cnt;
     if (!live) e1 = e0;
     const BT b = (live && sub == 0) ? ld_stream(rhs + row) : (BT)0;
-    XT acc = row_dot<VT, XT, L, true>(e0, e1, sub, ci, cv, x, err, coherent, base == 0 ? pc : 0);
+    XT acc = row_dot<VT, XT, L, true>(e0, e1, sub, ci, cv, x, err, coherent, base == 0 ? px : 0);
     acc = group_sum<L>(acc);
     if (live && sub == 0) {
       const XT v = ((XT)b - acc) / (XT)dg;
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(256) void k_sptrsv(
     const int info = n_info, m_s0 = n_s0;
     const int lg = (info >> kTaskInfoShift) & 15;
     const int cnt = info & ((1 << kTaskInfoShift) - 1);  // <= 64 slots (flat: 64 kFlatPerLane)
-    const int pc = info >> kTaskProxyShift;
+    const int px = info >> kTaskProxyShift;  // lane tasks' proxy (row_dot)
     const int m_row = n_row, m_e0 = n_e0, m_e1 = n_e1;
     const VT m_dg = n_dg;
     if (!checked && lg != kTaskFlat) {  // the first task that may wait
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(256) void k_sptrsv(
     if (k + 1 < k1) fetch(k + 1);
 #define RCHOL_SWEEP_TASK(LL)                                                                  \
   sweep_task<VT, XT, BT, LL, FUSE>(cnt, m_row, m_e0, m_e1, m_dg, ci, cv, rhs, x, aux, dotv, err, \
-                                   coherent, dacc, pc)
+                                   coherent, dacc, px)
     switch (lg) {
       case kTaskFlat:  // forward sweeps only (FUSE 0 / 1): keeps the r.z kernel's registers
         if constexpr (FUSE != 2)
